@@ -1,0 +1,63 @@
+// Internal host-side declarations shared by the library's translation units.
+#pragma once
+
+#include <atomic>
+
+#include "common.hpp"
+
+namespace b40c {
+
+// Owning device allocation (hipMalloc / hipFree).
+struct DeviceBuffer {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    ~DeviceBuffer() { release(); }
+    b40c_status alloc(size_t n) {
+        release();
+        if (n == 0) n = 1;
+        hipError_t e = hipMalloc(&ptr, n);
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            return fail(e == hipErrorOutOfMemory ? B40C_ERR_OOM : B40C_ERR_HIP, "hipMalloc(%zu) failed: %s", n,
+                        hipGetErrorString(e));
+        }
+        bytes = n;
+        return B40C_OK;
+    }
+    // Grow-only reallocation (contents are not preserved).
+    b40c_status reserve(size_t n) { return (n <= bytes && ptr) ? B40C_OK : alloc(n); }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    template <typename T>
+    T* as() const { return static_cast<T*>(ptr); }
+};
+
+// radix_sort.hip
+b40c_status radix_sort_u64(uint64_t* k0, uint64_t* k1, int64_t n, int begin_bit, int end_bit, int32_t* selector,
+                           hipStream_t stream);
+b40c_status radix_sort_pairs_u32(uint32_t* k0, uint32_t* k1, uint32_t* v0, uint32_t* v1, int64_t n, int begin_bit,
+                                 int end_bit, int32_t* selector, hipStream_t stream);
+
+}  // namespace b40c
+
+// Device-resident CSR graph (reference counted).
+struct b40c_graph {
+    int32_t nodes = 0;
+    int64_t edges = 0;
+    b40c::DeviceBuffer offsets;  // int32[nodes + 1]
+    b40c::DeviceBuffer cols;     // int32[edges]
+    std::atomic<int> refs{1};
+    const int32_t* d_offsets() const { return offsets.as<int32_t>(); }
+    const int32_t* d_cols() const { return cols.as<int32_t>(); }
+};
+
+namespace b40c {
+void graph_retain(b40c_graph* g);
+void graph_release(b40c_graph* g);
+}  // namespace b40c

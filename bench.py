@@ -1,0 +1,223 @@
+#!/usr/bin/env python3
+"""Benchmark: single-source BFS GTEPS on RMAT scale 22 on one MI355X.
+
+Workload: RMAT scale 22 (4,194,304 vertices), edge factor 16, (a,b,c,d) =
+(0.57, 0.19, 0.19, 0.05), symmetrized, self loops and duplicates removed,
+int32 CSR built on the GPU by radix-sorting packed 64-bit (src,dst) keys.
+64 seeded sources of nonzero degree.  One step = Reset + EnactSearch for the
+next source (inputs resident in HBM).  GTEPS = sum over steps of
+E_trav(src) / elapsed, E_trav = sum of degrees of reached vertices
+(test/bfs/test_bfs.cu:359-367).
+
+Prints one JSON line (see the README of the metric in the repo docs).
+"""
+import argparse
+import ctypes
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+class Hip:
+    """Minimal HIP event API over ctypes (timing on the problem's own stream)."""
+
+    def __init__(self):
+        self.L = ctypes.CDLL("libamdhip64.so")
+        self.L.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        self.L.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        self.L.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+        self.L.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
+        self.L.hipStreamSynchronize.argtypes = [ctypes.c_void_p]
+        self.L.hipDeviceSynchronize.argtypes = []
+
+    def event(self):
+        e = ctypes.c_void_p()
+        assert self.L.hipEventCreate(ctypes.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.L.hipEventRecord(e, stream) == 0
+
+    def elapsed_ms(self, e0, e1):
+        assert self.L.hipEventSynchronize(e1) == 0
+        ms = ctypes.c_float()
+        assert self.L.hipEventElapsedTime(ctypes.byref(ms), e0, e1) == 0
+        return ms.value
+
+    def sync(self):
+        assert self.L.hipDeviceSynchronize() == 0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def measure_strategy(bfs, b4, hip, graph, srcs, e_trav, v_vis, strategy, steps, warmup, mode=0):
+    prob = bfs.CsrProblem()
+    b4._lib.check(prob.FromGraph(graph))
+    en = bfs.ENACTORS[strategy]()
+    ft = en.GetFrontierType()
+    stream = prob.stream()
+    lib = b4.lib()
+
+    def step(k, evs=None):
+        s = int(srcs[k % len(srcs)])
+        b4._lib.check(prob.Reset(ft, 1.3))
+        if evs:
+            hip.record(evs[0], stream)
+        b4._lib.check(lib.b40c_enact_search(en.handle, prob.handle, s, 0, mode))
+        if evs:
+            hip.record(evs[1], stream)
+
+    for k in range(warmup):
+        step(k)
+    hip.sync()
+    ev = [(hip.event(), hip.event()) for _ in range(steps)]
+    t_all0, t_all1 = hip.event(), hip.event()
+    hip.sync()
+    hip.record(t_all0, stream)
+    w0 = time.perf_counter()
+    for k in range(steps):
+        step(warmup + k, ev[k])
+    hip.record(t_all1, stream)
+    total_ms = hip.elapsed_ms(t_all0, t_all1)
+    hip.sync()
+    wall_ms = (time.perf_counter() - w0) * 1e3
+    enact_ms = [hip.elapsed_ms(a, b) for a, b in ev]
+    idx = [(warmup + k) % len(srcs) for k in range(steps)]
+    et = np.array([e_trav[i] for i in idx], np.float64)
+    vv = np.array([v_vis[i] for i in idx], np.float64)
+    b_alg = 8.0 * et + 16.0 * vv
+    res = {
+        "gteps": float(et.sum() / (total_ms * 1e-3) / 1e9),
+        "gteps_enact_only": float(et.sum() / (np.sum(enact_ms) * 1e-3) / 1e9),
+        "ms_per_step": total_ms / steps,
+        "ms_per_search_enact": float(np.mean(enact_ms)),
+        "ms_per_search_p50": float(np.median(enact_ms)),
+        "host_wall_ms_per_step": wall_ms / steps,
+        "achieved_gbs": float(b_alg.sum() / (np.sum(enact_ms) * 1e-3) / 1e9),
+        "b_alg_per_search": float(b_alg.mean()),
+    }
+    prob.close()
+    en.close()
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=64)
+    ap.add_argument("--warmup", type=int, default=4)
+    ap.add_argument("--scale", type=int, default=22)
+    ap.add_argument("--sources", type=int, default=64)
+    ap.add_argument("--strategy", type=int, default=5, help="headline strategy (5 = direction-optimizing)")
+    ap.add_argument("--compare-steps", type=int, default=16, help="steps for the other strategies (0: skip)")
+    ap.add_argument("--cpu-sources", type=int, default=6, help="oracle BFS sources timed for cpu_baseline (0: skip)")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: every GPU box here has a single MI355X, and the "
+              "multi-GPU partitioned BFS is out of scope (--gpus %d refused)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import back40computing_amd as b4
+    from back40computing_amd import bfs
+    from back40computing_amd.sources import pick_sources
+
+    hip = Hip()
+    info = bfs.device_info()
+    log("device", info)
+
+    # ---- graph build on the GPU (timed separately; not part of a step) ----
+    hip.sync()
+    t0 = time.perf_counter()
+    graph = bfs.Graph.rmat(args.scale, 16, 0.57, 0.19, 0.19, 0)
+    hip.sync()
+    build_s = time.perf_counter() - t0
+    n, E = graph.nodes, graph.edges
+    log("rmat-%d built on GPU: V=%d E=%d in %.3f s" % (args.scale, n, E, build_s))
+    off = np.empty(n + 1, np.int32)
+    b4._lib.check(b4.lib().b40c_graph_copy_to_host(graph.handle, b4._lib.ptr(off), None))
+    deg = np.diff(off).astype(np.int64)
+    srcs = pick_sources(off, 1, args.sources)
+
+    # ---- untimed pass: reached vertices / traversed edges per source ------
+    e_trav, v_vis = [], []
+    for s in srcs:
+        depth, pred, st = bfs.bfs(graph, int(s), strategy=args.strategy)
+        m = depth >= 0
+        e_trav.append(int(deg[m].sum()))
+        v_vis.append(int(m.sum()))
+    log("per-source reached: mean V_vis=%.0f mean E_trav=%.0f" % (np.mean(v_vis), np.mean(e_trav)))
+
+    head = measure_strategy(bfs, b4, hip, graph, srcs, e_trav, v_vis, args.strategy, args.steps, args.warmup)
+    log("headline strategy %d: %s" % (args.strategy, json.dumps(head)))
+
+    names = {0: "expand_contract", 1: "contract_expand", 2: "two_phase", 3: "hybrid", 4: "persistent",
+             5: "direction_optimizing"}
+    per_strategy = {names[args.strategy]: round(head["gteps"], 3)}
+    if args.compare_steps > 0:
+        for s in (0, 1, 2, 3, 4, 5):
+            if s == args.strategy:
+                continue
+            r = measure_strategy(bfs, b4, hip, graph, srcs, e_trav, v_vis, s, args.compare_steps, 2)
+            per_strategy[names[s]] = round(r["gteps"], 3)
+            log("strategy %s: %.3f GTEPS (%.3f ms/search)" % (names[s], r["gteps"], r["ms_per_search_enact"]))
+
+    # ---- CPU baseline: the serial oracle BFS on a bounded sample ----------
+    cpu = None
+    if args.cpu_sources > 0:
+        import oracle
+        cols = np.empty(E, np.int32)
+        b4._lib.check(b4.lib().b40c_graph_copy_to_host(graph.handle, None, b4._lib.ptr(cols)))
+        tot_e, tot_ms = 0, 0.0
+        for s in srcs[:args.cpu_sources]:
+            d, sd, wall_ms, user_ms = oracle.bfs_timed(off, cols, int(s))
+            tot_e += int(deg[d >= 0].sum())
+            tot_ms += wall_ms
+        cpu = {"value": round(tot_e / (tot_ms * 1e-3) / 1e9, 5), "unit": "GTEPS", "cores": 1, "kind": "port",
+               "sample": "serial FIFO BFS (restated SimpleReferenceBfs) from the first %d of the %d sources on "
+                         "the same RMAT-%d CSR, steady_clock wall time, 1 thread" % (args.cpu_sources, len(srcs),
+                                                                                      args.scale),
+               "cpu": platform.processor() or platform.machine(), "nproc": os.cpu_count()}
+        log("cpu baseline", cpu)
+
+    achieved = head["achieved_gbs"]
+    out = {
+        "metric": "BFS GTEPS (RMAT scale-22) + batched-query p50/p99 vs plain, 1 MI355X",
+        "value": round(head["gteps"], 3),
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": "rmat%d_ef16_a0.57_b0.19_c0.19_sym_dedup_single_source_bfs" % args.scale,
+                   "vertices": n, "csr_edges": E, "sources": len(srcs), "strategy": names[args.strategy],
+                   "step": "Reset + EnactSearch, depth + predecessor output"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "cpu_baseline": cpu,
+        "detail": {"gteps_enact_only": round(head["gteps_enact_only"], 3),
+                   "ms_per_search_enact": round(head["ms_per_search_enact"], 4),
+                   "b_alg_per_search": head["b_alg_per_search"],
+                   "csr_build_s": round(build_s, 3), "per_strategy_gteps": per_strategy},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,178 @@
+/*
+ * b40c_mi355x — C ABI of the MI355X (gfx950) single-GPU BFS library.
+ *
+ * This is the drop-in boundary for b40c's single-GPU BFS path.  The reference
+ * is a header-only C++ template library (no extern "C", no shared library);
+ * each entry point below names the reference interface it replaces
+ * (file:line in the b40c tree).  All pointers are plain host pointers unless
+ * the name says d_ (device pointer).  No entry point throws; every one returns
+ * a b40c_status and b40c_last_error_string() describes the last failure on
+ * the calling thread.
+ *
+ * Conventions (identical to the reference): vertex ids, offsets and labels
+ * are int32; depth label -1 = unreached, source depth 0; predecessor -1 =
+ * unreached, source predecessor -2.
+ */
+#ifndef B40C_MI355X_H
+#define B40C_MI355X_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  B40C_ERR_QUEUE_OVERFLOW is the reference's
+ * cudaErrorInvalidConfiguration "Frontier queue overflow. Please increase
+ * queue-sizing factor" (b40c/graph/bfs/enactor_contract_expand.cuh:319-324);
+ * B40C_ERR_UNSUPPORTED its cudaErrorInvalidDeviceFunction for an untuned
+ * architecture (enactor_contract_expand.cuh:594-595). */
+typedef int32_t b40c_status;
+#define B40C_OK                 0
+#define B40C_ERR_QUEUE_OVERFLOW 1
+#define B40C_ERR_UNSUPPORTED    2
+#define B40C_ERR_OOM            3
+#define B40C_ERR_INVALID_ARG    4
+#define B40C_ERR_HIP            5
+#define B40C_ERR_ADMISSION      6
+
+/* Strategy numbering mirrors test/bfs/test_bfs.cu:72-79 (HOST = -1 is the
+ * CPU path and is not served by this library).  PERSISTENT and
+ * DIRECTION_OPTIMIZING have no counterpart in the reference enum. */
+#define B40C_EXPAND_CONTRACT      0
+#define B40C_CONTRACT_EXPAND      1
+#define B40C_TWO_PHASE            2
+#define B40C_HYBRID               3
+#define B40C_PERSISTENT           4
+#define B40C_DIRECTION_OPTIMIZING 5
+
+/* FrontierType (b40c/graph/bfs/csr_problem.cuh:54-61). */
+#define B40C_VERTEX_FRONTIERS 0
+#define B40C_EDGE_FRONTIERS   1
+#define B40C_MIXED_FRONTIERS  2
+
+/* Enactor launch modes: the strategy's own default (the form test_bfs.cu:
+ * 690-718 uses), the single persistent launch (EnactFusedSearch) or one
+ * launch per level (EnactIterativeSearch). */
+#define B40C_MODE_DEFAULT   0
+#define B40C_MODE_FUSED     1
+#define B40C_MODE_ITERATIVE 2
+
+typedef struct b40c_graph b40c_graph;
+typedef struct b40c_problem b40c_problem;
+typedef struct b40c_enactor b40c_enactor;
+
+/* Runtime tuning knobs.  They replace the compile-time KernelPolicy blocks
+ * the reference hand-sets per architecture (e.g. enactor_hybrid.cuh:569-655,
+ * contract_expand_atomic/kernel_policy.cuh:61-279).  Zero means "library
+ * default / autotuned". */
+typedef struct b40c_tuning {
+    int32_t grid_size;             /* workgroups per level launch (0: 4 x CUs)              */
+    int32_t cta_gather_threshold;  /* degree >= this: whole-workgroup gather                */
+    int32_t wave_gather_threshold; /* degree >= this: whole-wavefront gather                */
+    int32_t levels_per_sync;       /* iterative enactors: levels launched between host checks */
+    int32_t saturation_quit;       /* hybrid: fused kernel quits when frontier > grid * this */
+    int32_t do_alpha;              /* direction-optimizing: switch to bottom-up when m_f > m_u / alpha */
+    int32_t do_beta;               /* direction-optimizing: back to top-down when n_f < n / beta   */
+    int32_t persistent_wgs_per_cu; /* persistent kernels: resident workgroups per CU        */
+} b40c_tuning;
+
+/* ---- errors / device --------------------------------------------------- */
+const char* b40c_last_error_string(void);
+/* Reports device name, CU count and gfx architecture string. */
+b40c_status b40c_device_info(int32_t* num_cus, char* arch, int32_t arch_len);
+
+/* ---- device-resident CSR graphs ---------------------------------------- */
+/* Upload a host CSR (CsrProblem::FromHostProblem's H2D copy,
+ * csr_problem.cuh:328-403).  Host arrays stay caller-owned. */
+b40c_status b40c_graph_from_host_csr(int32_t nodes, int64_t edges, const int32_t* row_offsets,
+                                     const int32_t* column_indices, b40c_graph** out);
+/* GPU COO -> CSR with the reference's semantics: stable sort by row only,
+ * columns in input order (CsrGraph::FromCoo, b40c/graph/csr_graph.cuh:108-152),
+ * done with the GPU LSD radix sort instead of std::stable_sort.  Host COO in. */
+b40c_status b40c_graph_from_coo(int32_t nodes, int64_t edges, const int32_t* rows, const int32_t* cols,
+                                b40c_graph** out);
+/* Synthetic RMAT graph built on the GPU: 2^scale vertices, edge_factor*2^scale
+ * sampled edges, quadrant probabilities (a,b,c,1-a-b-c), symmetrized,
+ * self-loops and duplicates removed, int32 CSR from a radix sort of packed
+ * 64-bit (src,dst) keys (replaces BuildRmatGraph, graph/builder/rmat.cuh:127-206,
+ * and the host FromCoo sort). */
+b40c_status b40c_graph_build_rmat(int32_t scale, int32_t edge_factor, double a, double b, double c,
+                                  uint64_t seed, b40c_graph** out);
+/* Synthetic w^3 6-neighbour lattice with seeded random vertex relabelling and
+ * no self loops (replaces BuildGrid3dGraph, graph/builder/grid3d.cuh:53-127).
+ * perm (nullable, w^3 entries) receives original id -> new id. */
+b40c_status b40c_graph_build_lattice3d(int32_t width, uint64_t seed, int32_t* perm, b40c_graph** out);
+b40c_status b40c_graph_info(const b40c_graph* g, int32_t* nodes, int64_t* edges);
+b40c_status b40c_graph_copy_to_host(const b40c_graph* g, int32_t* row_offsets, int32_t* column_indices);
+/* Graphs are reference counted: problems hold a reference. */
+void b40c_graph_destroy(b40c_graph* g);
+
+/* ---- radix sort (b40c/radix_sort/enactor.cuh:153-266) ------------------- */
+/* Stable LSD sort of device keys over bits [begin_bit, end_bit) with 8-bit
+ * digits (upsweep -> spine -> downsweep per digit place).  d_keys[0] holds the
+ * input; d_keys[1] must have room for n keys.  On return *selector says which
+ * buffer holds the result (DoubleBuffer::selector, util/multiple_buffering.cuh:
+ * 65-80).  Digit places whose keys all share one digit are skipped. */
+b40c_status b40c_radix_sort_u64(uint64_t* d_keys0, uint64_t* d_keys1, int64_t n, int32_t begin_bit,
+                                int32_t end_bit, int32_t* selector);
+b40c_status b40c_radix_sort_pairs_u32(uint32_t* d_keys0, uint32_t* d_keys1, uint32_t* d_vals0,
+                                      uint32_t* d_vals1, int64_t n, int32_t begin_bit, int32_t end_bit,
+                                      int32_t* selector);
+/* Device allocation helpers so FFI callers can stage radix-sort buffers. */
+b40c_status b40c_device_alloc(void** d_ptr, int64_t bytes);
+b40c_status b40c_device_free(void* d_ptr);
+b40c_status b40c_memcpy_h2d(void* d_dst, const void* src, int64_t bytes);
+b40c_status b40c_memcpy_d2h(void* dst, const void* d_src, int64_t bytes);
+
+/* ---- problem (CsrProblem, b40c/graph/bfs/csr_problem.cuh) --------------- */
+/* FromHostProblem (csr_problem.cuh:328-334): copies the host CSR. */
+b40c_status b40c_problem_create(int32_t nodes, int64_t edges, const int32_t* column_indices,
+                                const int32_t* row_offsets, int32_t flags, b40c_problem** out);
+/* Same, over an already device-resident graph (no copy; takes a reference). */
+b40c_status b40c_problem_create_from_graph(b40c_graph* g, int32_t flags, b40c_problem** out);
+/* Reset (csr_problem.cuh:547-750): (re)allocate queues for frontier_type x
+ * queue_sizing and clear labels/visited state.  Must precede every search.
+ * queue_sizing <= 0 selects the reference default 1.3 (test_bfs.cu:802). */
+b40c_status b40c_problem_reset(b40c_problem* p, int32_t frontier_type, double queue_sizing);
+/* ExtractResults (csr_problem.cuh:255-322): depth and predecessor labels
+ * (either pointer may be NULL).  Synchronizes the problem's stream. */
+b40c_status b40c_problem_extract(b40c_problem* p, int32_t* depth, int32_t* pred);
+/* The HIP stream all of the problem's work runs on (hipStream_t). */
+void* b40c_problem_stream(b40c_problem* p);
+/* Device pointers of the labels (for callers that keep results resident). */
+b40c_status b40c_problem_device_labels(b40c_problem* p, int32_t** d_depth, int32_t** d_pred);
+void b40c_problem_destroy(b40c_problem* p);
+
+/* ---- enactors (EnactorContractExpand / ExpandContract / TwoPhase / Hybrid) */
+/* Enactor construction (enactor_*.cuh constructors; INSTRUMENT template flag). */
+b40c_status b40c_enactor_create(int32_t strategy, int32_t instrument, b40c_enactor** out);
+b40c_status b40c_enactor_set_tuning(b40c_enactor* e, const b40c_tuning* t);
+b40c_status b40c_enactor_get_tuning(const b40c_enactor* e, b40c_tuning* t);
+/* The frontier type a problem must be Reset with for this enactor
+ * (GetFrontierType, e.g. enactor_contract_expand.cuh:205). */
+int32_t b40c_enactor_frontier_type(const b40c_enactor* e);
+/* EnactSearch / EnactFusedSearch / EnactIterativeSearch from `src`
+ * (enactor_hybrid.cuh:559-563; enactor_contract_expand.cuh:338-342, 525-529).
+ * Asynchronous on the problem's stream except for the level-termination
+ * checks; returns B40C_ERR_QUEUE_OVERFLOW if any frontier overflowed. */
+b40c_status b40c_enact_search(b40c_enactor* e, b40c_problem* p, int32_t src, int32_t max_grid_size,
+                              int32_t mode);
+/* GetStatistics (enactor_contract_expand.cuh:236-250): total queued frontier
+ * elements, search depth (max label + 1) and average workgroup duty
+ * (instrumented enactors only, else 0). */
+b40c_status b40c_get_statistics(b40c_enactor* e, int64_t* total_queued, int32_t* search_depth,
+                                double* avg_duty);
+void b40c_enactor_destroy(b40c_enactor* e);
+
+/* ---- diagnostics ---------------------------------------------------------- */
+/* Times `iterations` back-to-back grid barriers of a cooperatively launched
+ * grid of `grid_size` workgroups (test/gbarrier/test_gbarrier.cu:43-113).
+ * Returns milliseconds per barrier in *ms_per_sync. */
+b40c_status b40c_gbarrier_benchmark(int32_t grid_size, int32_t iterations, double* ms_per_sync);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* B40C_MI355X_H */

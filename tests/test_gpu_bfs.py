@@ -1,0 +1,185 @@
+"""GPU parity: every strategy and launch mode against the serial oracle.
+
+Depth labels are compared bit-exactly; predecessors are checked with the
+reference's invariants (test/bfs/test_bfs.cu:387-438) plus source = -2;
+search depth must equal the oracle's max label + 1."""
+import numpy as np
+import pytest
+
+from helpers import csr_from_edges, hand_graphs
+
+pytestmark = pytest.mark.gpu
+
+import back40computing_amd as b4  # noqa: E402
+
+STRATEGY_MODES = [
+    (b4.EXPAND_CONTRACT, b4.MODE_ITERATIVE),
+    (b4.EXPAND_CONTRACT, b4.MODE_FUSED),
+    (b4.CONTRACT_EXPAND, b4.MODE_ITERATIVE),
+    (b4.CONTRACT_EXPAND, b4.MODE_FUSED),
+    (b4.TWO_PHASE, b4.MODE_ITERATIVE),
+    (b4.TWO_PHASE, b4.MODE_FUSED),
+    (b4.HYBRID, b4.MODE_DEFAULT),
+    (b4.PERSISTENT, b4.MODE_DEFAULT),
+    (b4.DIRECTION_OPTIMIZING, b4.MODE_DEFAULT),
+]
+SM_IDS = ["%d-%d" % sm for sm in STRATEGY_MODES]
+
+
+@pytest.fixture(scope="module")
+def bfsmod():
+    from back40computing_amd import bfs
+    return bfs
+
+
+def run(bfsmod, graph, src, strategy, mode, tuning=None, queue_sizing=1.3):
+    print("bfs strategy=%d mode=%d src=%d n=%d" % (strategy, mode, src, graph.nodes), flush=True)
+    return bfsmod.bfs(graph, src, strategy=strategy, mode=mode, tuning=tuning, queue_sizing=queue_sizing)
+
+
+def check(orc, off, cols, src, depth, pred, stats):
+    ref, _, sd = orc.bfs(off, cols, int(src), with_pred=False)
+    np.testing.assert_array_equal(depth, ref)
+    assert orc.check_pred(off, cols, ref, pred) == -1
+    assert stats["search_depth"] == sd
+
+
+@pytest.fixture(scope="module")
+def rmat16(bfsmod, orc):
+    g = bfsmod.Graph.rmat(16, 16, 0.57, 0.19, 0.19, 0)
+    off, cols = g.to_host()
+    srcs = orc.pick_sources(off, 1, 3)
+    return g, off, cols, srcs
+
+
+@pytest.mark.parametrize("g", hand_graphs(), ids=lambda g: g["name"])
+@pytest.mark.parametrize("sm", STRATEGY_MODES, ids=SM_IDS)
+def test_hand_graphs(bfsmod, orc, g, sm):
+    off, cols = csr_from_edges(g["n"], g["edges"], g["undirected"])
+    graph = bfsmod.Graph.from_host_csr(off, cols)
+    for case in g["cases"]:
+        depth, pred, st = run(bfsmod, graph, case["src"], *sm)
+        assert depth.tolist() == case["depth"]
+        assert st["search_depth"] == case["search_depth"]
+        assert orc.check_pred(off, cols, depth, pred) == -1
+
+
+@pytest.mark.parametrize("sm", STRATEGY_MODES, ids=SM_IDS)
+def test_rmat16_all_strategies(bfsmod, orc, rmat16, sm):
+    g, off, cols, srcs = rmat16
+    for s in srcs:
+        depth, pred, st = run(bfsmod, g, int(s), *sm)
+        check(orc, off, cols, s, depth, pred, st)
+
+
+@pytest.mark.parametrize("sm", STRATEGY_MODES, ids=SM_IDS)
+def test_lattice_closed_form(bfsmod, orc, sm):
+    w = 24
+    g, perm = bfsmod.Graph.lattice3d(w, 5, want_perm=True)
+    off, cols = g.to_host()
+    for s in (0, w ** 3 // 2 + 7):
+        depth, pred, st = run(bfsmod, g, s, *sm)
+        np.testing.assert_array_equal(depth, orc.lattice_depth(w, perm, s))
+        assert orc.check_pred(off, cols, depth, pred) == -1
+
+
+@pytest.mark.parametrize("sm", STRATEGY_MODES, ids=SM_IDS)
+def test_gather_tiers(bfsmod, orc, sm):
+    # A star whose hub exceeds the workgroup tier, mid-degree vertices for the
+    # wave tier and a long tail for the fine tier; thresholds lowered so each
+    # tier is exercised on a small graph.
+    rng = np.random.default_rng(3)
+    n = 20000
+    edges = [(0, i) for i in range(1, 9000)]
+    for h in range(9000, 9100):
+        edges += [(h, int(x)) for x in rng.integers(9100, n, 100)]
+    edges += [(int(a), int(b)) for a, b in rng.integers(1, n, (30000, 2))]
+    off, cols = csr_from_edges(n, edges, undirected=True)
+    g = bfsmod.Graph.from_host_csr(off, cols)
+    for tuning in (None, {"cta_gather_threshold": 300, "wave_gather_threshold": 8}):
+        for s in (0, 9050, 15000):
+            depth, pred, st = run(bfsmod, g, s, *sm, tuning=tuning)
+            check(orc, off, cols, s, depth, pred, st)
+
+
+@pytest.mark.parametrize("sm", STRATEGY_MODES, ids=SM_IDS)
+def test_reuse_problem_across_sources(bfsmod, orc, rmat16, sm):
+    g, off, cols, srcs = rmat16
+    strategy, mode = sm
+    prob = bfsmod.CsrProblem()
+    assert prob.FromGraph(g) == 0
+    en = bfsmod.ENACTORS[strategy]()
+    for s in srcs:
+        assert prob.Reset(en.GetFrontierType(), 1.3) == 0
+        assert b4.lib().b40c_enact_search(en.handle, prob.handle, int(s), 0, mode) == 0
+        depth, pred = prob.results()
+        q, sd, duty = en.GetStatistics()
+        check(orc, off, cols, s, depth, pred, {"search_depth": sd})
+        assert q > 0
+
+
+def test_enact_requires_reset(bfsmod, rmat16):
+    g, off, cols, srcs = rmat16
+    prob = bfsmod.CsrProblem()
+    assert prob.FromGraph(g) == 0
+    en = bfsmod.EnactorHybrid()
+    assert en.EnactSearch(prob, int(srcs[0])) == b4._lib.ERR_INVALID_ARG
+    assert prob.Reset(en.GetFrontierType()) == 0
+    assert en.EnactSearch(prob, -1) == b4._lib.ERR_INVALID_ARG
+    assert en.EnactSearch(prob, int(srcs[0])) == 0
+    # a second search without Reset is refused
+    assert en.EnactSearch(prob, int(srcs[0])) == b4._lib.ERR_INVALID_ARG
+    # a strategy needing edge frontiers refuses a vertex-frontier reset
+    assert prob.Reset(b4.VERTEX_FRONTIERS) == 0
+    assert bfsmod.EnactorContractExpand().EnactSearch(prob, int(srcs[0])) == b4._lib.ERR_INVALID_ARG
+
+
+@pytest.mark.parametrize("strategy,mode", [(b4.CONTRACT_EXPAND, b4.MODE_ITERATIVE),
+                                           (b4.CONTRACT_EXPAND, b4.MODE_FUSED),
+                                           (b4.TWO_PHASE, b4.MODE_ITERATIVE),
+                                           (b4.HYBRID, b4.MODE_DEFAULT),
+                                           (b4.EXPAND_CONTRACT, b4.MODE_ITERATIVE)])
+def test_forced_queue_overflow(bfsmod, rmat16, strategy, mode):
+    """A tiny queue-sizing factor must surface as QUEUE_OVERFLOW, never as
+    silent corruption (enactor_contract_expand.cuh:319-324)."""
+    g, off, cols, srcs = rmat16
+    with pytest.raises(b4.B40CError) as ei:
+        run(bfsmod, g, int(srcs[0]), strategy, mode, queue_sizing=0.001)
+    assert ei.value.status == b4._lib.ERR_QUEUE_OVERFLOW
+
+
+def test_statistics_and_instrumentation(bfsmod, orc, rmat16):
+    g, off, cols, srcs = rmat16
+    for cls in (bfsmod.EnactorExpandContract, bfsmod.EnactorTwoPhase, bfsmod.EnactorPersistent):
+        prob = bfsmod.CsrProblem()
+        assert prob.FromGraph(g) == 0
+        en = cls(instrument=True)
+        assert prob.Reset(en.GetFrontierType()) == 0
+        assert en.EnactSearch(prob, int(srcs[0])) == 0
+        depth, pred = prob.results()
+        q, sd, duty = en.GetStatistics()
+        ref, _, rsd = orc.bfs(off, cols, int(srcs[0]), with_pred=False)
+        np.testing.assert_array_equal(depth, ref)
+        assert sd == rsd
+        assert 0.0 < duty <= 1.0 + 1e-9
+        assert q >= int((ref >= 0).sum())
+
+
+def test_gbarrier_benchmark(bfsmod):
+    info = bfsmod.device_info()
+    ms = bfsmod.gbarrier_benchmark(info["num_cus"], 1000)
+    assert 0 < ms < 1.0
+
+
+def test_from_host_problem_signature(bfsmod, orc):
+    off, cols = orc.build_rmat_csr(10, 16, seed=4)
+    prob = bfsmod.CsrProblem()
+    assert prob.FromHostProblem(False, off.size - 1, cols.size, cols, off, 1) == 0
+    en = bfsmod.EnactorHybrid()
+    src = int(orc.pick_sources(off, 3, 1)[0])
+    assert prob.Reset(en.GetFrontierType(), 1.3) == 0
+    assert en.EnactSearch(prob, src) == 0
+    labels = np.empty(off.size - 1, np.int32)
+    assert prob.ExtractResults(labels) == 0
+    np.testing.assert_array_equal(labels, orc.bfs(off, cols, src, with_pred=False)[0])
+    assert prob.FromHostProblem(False, off.size - 1, cols.size, cols, off, 2) == b4._lib.ERR_UNSUPPORTED

@@ -1,0 +1,100 @@
+"""GPU parity: radix sort and CSR builders against the oracle (bit-exact)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def rs():
+    from back40computing_amd import radix_sort
+    return radix_sort
+
+
+@pytest.fixture(scope="module")
+def G():
+    from back40computing_amd.bfs import Graph
+    return Graph
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 63, 2048, 2049, 100003, 1 << 20])
+def test_sort_u64_random(rs, orc, n):
+    rng = np.random.default_rng(n)
+    k = rng.integers(0, 2**64 - 1, n, dtype=np.uint64)
+    print("sort_u64 n=%d" % n, flush=True)
+    np.testing.assert_array_equal(rs.sort_u64(k), orc.sort_u64(k))
+
+
+def test_sort_u64_bit_ranges_and_trivial_passes(rs, orc):
+    rng = np.random.default_rng(7)
+    # 20-bit keys with the same high bits: passes over bits 24..63 are trivial.
+    k = (rng.integers(0, 1 << 20, 300000).astype(np.uint64) | np.uint64(0xAB00000000000000))
+    np.testing.assert_array_equal(rs.sort_u64(k), orc.sort_u64(k))
+    # Sorting only bits [0, 12) keeps the order of keys that tie on those bits.
+    k2 = rng.integers(0, 1 << 16, 50000).astype(np.uint64)
+    out = rs.sort_u64(k2, 0, 12)
+    order = np.argsort(k2 & np.uint64(0xFFF), kind="stable")
+    np.testing.assert_array_equal(out, k2[order])
+    # All keys identical.
+    k3 = np.full(10000, 12345, np.uint64)
+    np.testing.assert_array_equal(rs.sort_u64(k3), k3)
+
+
+@pytest.mark.parametrize("n", [1, 5000, 300001])
+def test_sort_pairs_u32_is_stable(rs, orc, n):
+    rng = np.random.default_rng(n)
+    keys = rng.integers(0, 1000, n).astype(np.uint32)
+    vals = np.arange(n, dtype=np.uint32)
+    sk, sv = rs.sort_pairs_u32(keys, vals, 0, 10)
+    ok, ov = orc.stable_sort_pairs_u32(keys, vals)
+    np.testing.assert_array_equal(sk, ok)
+    np.testing.assert_array_equal(sv, ov)
+
+
+@pytest.mark.parametrize("scale,ef,seed", [(6, 4, 1), (10, 16, 0), (14, 16, 5), (16, 16, 0)])
+def test_rmat_csr_bit_exact(G, orc, scale, ef, seed):
+    print("rmat scale=%d ef=%d seed=%d" % (scale, ef, seed), flush=True)
+    g = G.rmat(scale, ef, 0.57, 0.19, 0.19, seed)
+    off, cols = g.to_host()
+    ref_off, ref_cols = orc.build_rmat_csr(scale, ef, 0.57, 0.19, 0.19, seed)
+    np.testing.assert_array_equal(off, ref_off)
+    np.testing.assert_array_equal(cols, ref_cols)
+    g.close()
+
+
+def test_rmat_other_probabilities(G, orc):
+    g = G.rmat(12, 8, 0.45, 0.15, 0.15, 3)
+    off, cols = g.to_host()
+    ref_off, ref_cols = orc.build_rmat_csr(12, 8, 0.45, 0.15, 0.15, 3)
+    np.testing.assert_array_equal(off, ref_off)
+    np.testing.assert_array_equal(cols, ref_cols)
+
+
+@pytest.mark.parametrize("w,seed", [(1, 0), (2, 3), (8, 1), (30, 7)])
+def test_lattice_csr_bit_exact(G, orc, w, seed):
+    g, perm = G.lattice3d(w, seed, want_perm=True)
+    off, cols = g.to_host()
+    ref_off, ref_cols, ref_perm = orc.build_lattice_csr(w, seed)
+    np.testing.assert_array_equal(perm, ref_perm)
+    np.testing.assert_array_equal(off, ref_off)
+    np.testing.assert_array_equal(cols, ref_cols)
+
+
+@pytest.mark.parametrize("n,m", [(1, 0), (7, 30), (1000, 20000), (50000, 400000)])
+def test_from_coo_matches_reference_stable_order(G, orc, n, m):
+    rng = np.random.default_rng(m)
+    rows = rng.integers(0, n, m).astype(np.int32)
+    cols = rng.integers(0, n, m).astype(np.int32)
+    g = G.from_coo(n, rows, cols)
+    off, out = g.to_host()
+    ref_off, ref_out = orc.csr_from_coo(n, rows, cols)
+    np.testing.assert_array_equal(off, ref_off)
+    np.testing.assert_array_equal(out, ref_out)
+
+
+def test_host_csr_round_trip(G, orc):
+    off, cols = orc.build_rmat_csr(9, 8, seed=2)
+    g = G.from_host_csr(off, cols)
+    o2, c2 = g.to_host()
+    np.testing.assert_array_equal(o2, off)
+    np.testing.assert_array_equal(c2, cols)
