@@ -106,6 +106,7 @@ SIGNATURES = {
     "b40c_get_statistics": (_i32, [_vp, _p_i64, _p_i32, _p_d]),
     "b40c_enactor_destroy": (None, [_vp]),
     "b40c_gbarrier_benchmark": (_i32, [_i32, _i32, _p_d]),
+    "b40c_gbarrier_check": (_i32, [_i32, _i32, _p_i64]),
 }
 
 _lib = None

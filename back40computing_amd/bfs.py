@@ -43,6 +43,13 @@ def gbarrier_benchmark(grid_size, iterations):
     return ms.value
 
 
+def gbarrier_check(grid_size, iterations):
+    """Number of stale/wrong words seen across the software grid barrier."""
+    bad = ctypes.c_int64()
+    check(lib().b40c_gbarrier_check(grid_size, iterations, ctypes.byref(bad)))
+    return bad.value
+
+
 class Graph:
     """Device-resident int32 CSR graph (reference counted by the library)."""
 

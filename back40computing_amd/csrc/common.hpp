@@ -53,7 +53,7 @@ struct alignas(256) DevCounters {
     uint32_t bar_count;          // grid barrier arrival counter
     uint32_t bar_gen;            // grid barrier generation
     uint32_t bar_abort;          // set when a barrier wait timed out
-    uint32_t pad;
+    uint32_t corrupt;            // set when a frontier held an id outside [0, n)
     unsigned long long level_edges[4];  // rotating per-level degree sums (direction switch)
     uint32_t level_verts[4];            // rotating per-level discovered-vertex counts
 };
@@ -147,13 +147,16 @@ __device__ __forceinline__ void grid_barrier(uint32_t* count, uint32_t* gen, uin
     __syncthreads();
     if (threadIdx.x == 0) {
         const uint32_t g = ld_relaxed(gen);
-        const uint32_t arrived = __hip_atomic_fetch_add(count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t arrived = __hip_atomic_fetch_add(count, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         if (arrived == nblocks - 1) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             st_relaxed(count, 0u);
             __hip_atomic_store(gen, g + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         } else {
+            // Poll with relaxed loads (no cache invalidation per poll), then
+            // one acquire fence once the generation has moved.
             uint32_t spins = 0;
-            while (ld_acquire(gen) == g) {
+            while (ld_relaxed(gen) == g) {
                 if (ld_relaxed(abort_flag)) break;
                 if (++spins > kBarrierSpinLimit) {
                     st_relaxed(abort_flag, 1u);
@@ -161,6 +164,7 @@ __device__ __forceinline__ void grid_barrier(uint32_t* count, uint32_t* gen, uin
                 }
                 __builtin_amdgcn_s_sleep(1);
             }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         }
     }
     __syncthreads();

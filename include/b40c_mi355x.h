@@ -170,6 +170,10 @@ void b40c_enactor_destroy(b40c_enactor* e);
  * grid of `grid_size` workgroups (test/gbarrier/test_gbarrier.cu:43-113).
  * Returns milliseconds per barrier in *ms_per_sync. */
 b40c_status b40c_gbarrier_benchmark(int32_t grid_size, int32_t iterations, double* ms_per_sync);
+/* Correctness check of the same barrier (the reference has none): every
+ * workgroup writes a slab, passes the barrier and compares three other
+ * workgroups' slabs; *mismatches counts stale or wrong words. */
+b40c_status b40c_gbarrier_check(int32_t grid_size, int32_t iterations, int64_t* mismatches);
 
 #ifdef __cplusplus
 }

@@ -12,7 +12,11 @@ pytestmark = pytest.mark.gpu
 
 import back40computing_amd as b4  # noqa: E402
 
-STRATEGY_MODES = [
+# The persistent (cooperative, grid-barrier) kernels are switched off in the
+# library until the cause of a GPU memory fault seen in a run that used them
+# is shown; their cases are skipped and the refusal itself is tested below.
+FUSED_SKIP = pytest.mark.skip(reason="persistent kernels disabled pending the cause of a GPU fault")
+ALL_MODES = [
     (b4.EXPAND_CONTRACT, b4.MODE_ITERATIVE),
     (b4.EXPAND_CONTRACT, b4.MODE_FUSED),
     (b4.CONTRACT_EXPAND, b4.MODE_ITERATIVE),
@@ -23,7 +27,25 @@ STRATEGY_MODES = [
     (b4.PERSISTENT, b4.MODE_DEFAULT),
     (b4.DIRECTION_OPTIMIZING, b4.MODE_DEFAULT),
 ]
-SM_IDS = ["%d-%d" % sm for sm in STRATEGY_MODES]
+
+
+def is_fused(sm):
+    return sm[1] == b4.MODE_FUSED or sm[0] == b4.PERSISTENT
+
+
+def params(held_back=()):
+    out = []
+    for sm in ALL_MODES:
+        marks = []
+        if is_fused(sm):
+            marks.append(FUSED_SKIP)
+        elif sm in held_back:
+            marks.append(pytest.mark.skip(reason="held back pending the cause of a GPU fault"))
+        out.append(pytest.param(sm, marks=marks, id="%d-%d" % sm))
+    return out
+
+
+STRATEGY_MODES = params()
 
 
 @pytest.fixture(scope="module")
@@ -53,7 +75,7 @@ def rmat16(bfsmod, orc):
 
 
 @pytest.mark.parametrize("g", hand_graphs(), ids=lambda g: g["name"])
-@pytest.mark.parametrize("sm", STRATEGY_MODES, ids=SM_IDS)
+@pytest.mark.parametrize("sm", STRATEGY_MODES)
 def test_hand_graphs(bfsmod, orc, g, sm):
     off, cols = csr_from_edges(g["n"], g["edges"], g["undirected"])
     graph = bfsmod.Graph.from_host_csr(off, cols)
@@ -64,7 +86,7 @@ def test_hand_graphs(bfsmod, orc, g, sm):
         assert orc.check_pred(off, cols, depth, pred) == -1
 
 
-@pytest.mark.parametrize("sm", STRATEGY_MODES, ids=SM_IDS)
+@pytest.mark.parametrize("sm", STRATEGY_MODES)
 def test_rmat16_all_strategies(bfsmod, orc, rmat16, sm):
     g, off, cols, srcs = rmat16
     for s in srcs:
@@ -72,7 +94,12 @@ def test_rmat16_all_strategies(bfsmod, orc, rmat16, sm):
         check(orc, off, cols, s, depth, pred, st)
 
 
-@pytest.mark.parametrize("sm", STRATEGY_MODES, ids=SM_IDS)
+# Two-phase on the lattice is one of the two cases that may have faulted the
+# GPU (the run's log was lost with the box); it stays off until a cause is shown.
+LATTICE_MODES = params(held_back=[(b4.TWO_PHASE, b4.MODE_ITERATIVE)])
+
+
+@pytest.mark.parametrize("sm", LATTICE_MODES)
 def test_lattice_closed_form(bfsmod, orc, sm):
     w = 24
     g, perm = bfsmod.Graph.lattice3d(w, 5, want_perm=True)
@@ -83,7 +110,7 @@ def test_lattice_closed_form(bfsmod, orc, sm):
         assert orc.check_pred(off, cols, depth, pred) == -1
 
 
-@pytest.mark.parametrize("sm", STRATEGY_MODES, ids=SM_IDS)
+@pytest.mark.parametrize("sm", STRATEGY_MODES)
 def test_gather_tiers(bfsmod, orc, sm):
     # A star whose hub exceeds the workgroup tier, mid-degree vertices for the
     # wave tier and a long tail for the fine tier; thresholds lowered so each
@@ -102,7 +129,7 @@ def test_gather_tiers(bfsmod, orc, sm):
             check(orc, off, cols, s, depth, pred, st)
 
 
-@pytest.mark.parametrize("sm", STRATEGY_MODES, ids=SM_IDS)
+@pytest.mark.parametrize("sm", STRATEGY_MODES)
 def test_reuse_problem_across_sources(bfsmod, orc, rmat16, sm):
     g, off, cols, srcs = rmat16
     strategy, mode = sm
@@ -135,7 +162,6 @@ def test_enact_requires_reset(bfsmod, rmat16):
 
 
 @pytest.mark.parametrize("strategy,mode", [(b4.CONTRACT_EXPAND, b4.MODE_ITERATIVE),
-                                           (b4.CONTRACT_EXPAND, b4.MODE_FUSED),
                                            (b4.TWO_PHASE, b4.MODE_ITERATIVE),
                                            (b4.HYBRID, b4.MODE_DEFAULT),
                                            (b4.EXPAND_CONTRACT, b4.MODE_ITERATIVE)])
@@ -150,7 +176,7 @@ def test_forced_queue_overflow(bfsmod, rmat16, strategy, mode):
 
 def test_statistics_and_instrumentation(bfsmod, orc, rmat16):
     g, off, cols, srcs = rmat16
-    for cls in (bfsmod.EnactorExpandContract, bfsmod.EnactorTwoPhase, bfsmod.EnactorPersistent):
+    for cls in (bfsmod.EnactorExpandContract, bfsmod.EnactorTwoPhase, bfsmod.EnactorContractExpand):
         prob = bfsmod.CsrProblem()
         assert prob.FromGraph(g) == 0
         en = cls(instrument=True)
@@ -165,9 +191,37 @@ def test_statistics_and_instrumentation(bfsmod, orc, rmat16):
         assert q >= int((ref >= 0).sum())
 
 
-def test_gbarrier_benchmark(bfsmod):
+def test_fused_modes_refused(bfsmod, rmat16):
+    """While the persistent kernels are off, every fused request is refused
+    on the host with UNSUPPORTED (no kernel is launched)."""
+    g, off, cols, srcs = rmat16
+    for cls in (bfsmod.EnactorExpandContract, bfsmod.EnactorContractExpand, bfsmod.EnactorTwoPhase):
+        prob = bfsmod.CsrProblem()
+        assert prob.FromGraph(g) == 0
+        en = cls()
+        assert prob.Reset(en.GetFrontierType()) == 0
+        assert en.EnactFusedSearch(prob, int(srcs[0])) == b4._lib.ERR_UNSUPPORTED
+    prob = bfsmod.CsrProblem()
+    assert prob.FromGraph(g) == 0
+    en = bfsmod.EnactorPersistent()
+    assert prob.Reset(en.GetFrontierType()) == 0
+    assert en.EnactSearch(prob, int(srcs[0])) == b4._lib.ERR_UNSUPPORTED
+
+
+def test_gbarrier_zz_visibility(bfsmod):
+    """Words written before the software grid barrier are seen by other
+    workgroups after it (indices are fixed and in bounds; values are only
+    compared)."""
+    info = bfsmod.device_info()
+    for grid in (info["num_cus"], 2 * info["num_cus"]):
+        print("gbarrier_check grid=%d" % grid, flush=True)
+        assert bfsmod.gbarrier_check(grid, 200) == 0
+
+
+def test_gbarrier_zz_benchmark(bfsmod):
     info = bfsmod.device_info()
     ms = bfsmod.gbarrier_benchmark(info["num_cus"], 1000)
+    print("gbarrier %.4f ms/sync at grid %d" % (ms, info["num_cus"]), flush=True)
     assert 0 < ms < 1.0
 
 
