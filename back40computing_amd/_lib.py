@@ -58,6 +58,7 @@ class Tuning(ctypes.Structure):
         ("do_alpha", ctypes.c_int32),
         ("do_beta", ctypes.c_int32),
         ("persistent_wgs_per_cu", ctypes.c_int32),
+        ("grid_gather_threshold", ctypes.c_int32),
     ]
 
     def as_dict(self):

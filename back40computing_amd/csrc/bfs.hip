@@ -1,5 +1,5 @@
-// Single-GPU BFS for gfx950: level kernels, persistent (cooperative) kernels,
-// the CsrProblem object and the enactors.
+// Single-GPU BFS for gfx950: level kernels, the grid-tier kernel, persistent
+// (cooperative) kernels, the CsrProblem object and the enactors.
 //
 // Reference counterparts (b40c):
 //   CsrProblem                b40c/graph/bfs/csr_problem.cuh
@@ -16,7 +16,13 @@
 // slot (ph+2)&3.  Expand-contract and contract-expand use one phase per BFS
 // level, two-phase uses two (contract, expand) or three (filter, contract,
 // expand).  Depth labels are carried separately from the phase index.
+//
+// Grid tier: an expanding phase defers vertices whose degree reaches
+// grid_th to a list (length slot ph&1); the k_heavy launch that follows the
+// phase splits those adjacency lists into HEAVY_CHUNK-edge pieces spread over
+// every workgroup, so one hub no longer serializes a level on one workgroup.
 #include <algorithm>
+#include <climits>
 #include <cstring>
 #include <vector>
 
@@ -49,6 +55,15 @@ __device__ __forceinline__ void shared_init(LevelShared& s) {
     __syncthreads();
 }
 
+// Frontier entries are validated before they are used as addresses: an id
+// outside [0, n) is dropped and flagged, so a corrupt queue surfaces as an
+// error status instead of an out-of-bounds access.
+__device__ __forceinline__ bool valid_vertex(const BfsArgs& a, int32_t v) {
+    if ((uint32_t)v < (uint32_t)a.n) return true;
+    st_relaxed(&a.ctr->corrupt, 1u);
+    return false;
+}
+
 __device__ __forceinline__ uint32_t phase_input_len(const BfsArgs& a, int ph) {
     uint32_t n = ld_relaxed(&a.ctr->qlen[ph & 3]);
     if (n > a.qcap[ph & 1]) n = 0;  // an overflowed queue counts as empty
@@ -58,6 +73,7 @@ __device__ __forceinline__ uint32_t phase_input_len(const BfsArgs& a, int ph) {
 __device__ __forceinline__ void phase_begin(const BfsArgs& a, int ph, uint32_t n_in) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         st_relaxed(&a.ctr->qlen[(ph + 2) & 3], 0u);
+        __hip_atomic_store(&a.ctr->heavy_alloc[(ph + 1) & 1], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         a.ctr->level_edges[(ph + 2) & 3] = 0ull;
         if (n_in) atomic_add_u64(&a.ctr->queued, n_in);
     }
@@ -71,15 +87,6 @@ __device__ __forceinline__ GlobalQueue out_queue(const BfsArgs& a, int ph) {
     q.cap = a.qcap[(ph + 1) & 1];
     q.overflow = &a.ctr->overflow;
     return q;
-}
-
-// Frontier entries are validated before they are used as addresses: an id
-// outside [0, n) is dropped and flagged, so a corrupt queue surfaces as an
-// error status instead of an out-of-bounds access.
-__device__ __forceinline__ bool valid_vertex(const BfsArgs& a, int32_t v) {
-    if ((uint32_t)v < (uint32_t)a.n) return true;
-    st_relaxed(&a.ctr->corrupt, 1u);
-    return false;
 }
 
 // Workgroup sum of a per-thread u64, added once to *dst.
@@ -96,19 +103,90 @@ __device__ __forceinline__ void wg_add_u64(unsigned long long v, unsigned long l
     }
 }
 
+// Grid-tier deferral (wave-uniform): lanes whose vertex has degree >= grid_th
+// hand its adjacency list to the grid tier of phase ph and drop their own
+// expansion.  One 64-bit atomic per wave allocates, in the same order, entry
+// slots (high word) and HEAVY_CHUNK-edge chunk ranges (low word); each lane
+// records (vertex, first chunk) and points its chunks at its entry, so the
+// grid-tier kernel maps any chunk to its vertex with one load.
+__device__ __forceinline__ void defer_heavy(const BfsArgs& a, int ph, int32_t v, int32_t& deg) {
+    const bool heavy = deg >= a.grid_th;
+    const uint64_t m = __ballot(heavy);
+    if (m == 0) return;
+    const int lane = lane_id();
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    const uint32_t nch = heavy ? (uint32_t)((deg + HEAVY_CHUNK - 1) / HEAVY_CHUNK) : 0u;
+    const uint32_t ch_incl = (uint32_t)wave_inclusive_scan((int)nch);
+    const uint32_t ch_total = (uint32_t)__shfl((int)ch_incl, kWave - 1, kWave);
+    unsigned long long base = 0;
+    if (lane == leader)
+        base = __hip_atomic_fetch_add(&a.ctr->heavy_alloc[ph & 1],
+                                      ((unsigned long long)__popcll(m) << 32) | (unsigned long long)ch_total,
+                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t base_lo = (uint32_t)wave_bcast((int)(uint32_t)base, leader);
+    const uint32_t base_hi = (uint32_t)wave_bcast((int)(uint32_t)(base >> 32), leader);
+    if (heavy) {
+        const uint32_t entry = base_hi + (uint32_t)lane_prefix(m);
+        const uint32_t c0 = base_lo + ch_incl - nch;
+        if (entry < a.heavy_cap && c0 + nch <= a.chunk_cap) {
+            a.heavy[entry] = make_int2(v, (int)c0);
+            for (uint32_t c = 0; c < nch; ++c) a.chunk_map[c0 + c] = (int32_t)entry;
+        } else {
+            st_relaxed(&a.ctr->overflow, 1u);
+        }
+        deg = 0;
+    }
+}
+
+// Top-down discovery sink: exact claim, labels, optional frontier bitmap bit
+// and degree accounting for the direction switch, then the LDS queue.
+struct TdSink {
+    const BfsArgs& a;
+    LevelShared& s;
+    const GlobalQueue& out;
+    uint32_t* next_bm;
+    int32_t depth_next;
+    bool count_edges;
+    bool found;
+    unsigned long long eacc;
+    __device__ __forceinline__ void operator()(bool valid, int32_t v, int32_t src) {
+        const bool won = valid && claim_exact(a.visited, v);
+        if (won) {
+            a.depth[v] = depth_next;
+            a.pred[v] = src;
+            found = true;
+            if (next_bm) atomic_or_u32(&next_bm[v >> 5], 1u << (v & 31));
+            if (count_edges) eacc += (unsigned long long)(a.offs[v + 1] - a.offs[v]);
+        }
+        lq_append(s.qv, out, won, v, 0);
+    }
+};
+
+struct PairSink {
+    LevelShared& s;
+    const GlobalQueue& out;
+    __device__ __forceinline__ void operator()(bool valid, int32_t nbr, int32_t src) {
+        lq_append(s.qp, out, valid, nbr, src);
+    }
+};
+
+__device__ __forceinline__ void td_finish(const BfsArgs& a, int ph, TdSink& sink, LevelShared& s) {
+    lq_flush(s.qv, sink.out, 1);
+    if (sink.count_edges) wg_add_u64(sink.eacc, &a.ctr->level_edges[(ph + 1) & 3], s);
+    if (__syncthreads_or(sink.found) && threadIdx.x == 0) atomic_max_u32(a.max_depth, (uint32_t)sink.depth_next);
+}
+
 // Expand-contract (vertex frontier -> vertex frontier).  Neighbours are
 // culled and labelled at discovery (depth = depth_next), as in
 // expand_contract_atomic/cta.cuh:212-214, and compacted into the next vertex
-// frontier.  With COUNT_EDGES the degree sum of the discovered vertices is
-// accumulated for the direction-optimizing switch.
-template <bool COUNT_EDGES>
-__device__ void td_phase(const BfsArgs& a, int ph, int32_t depth_next, LevelShared& s) {
+// frontier.
+__device__ void td_phase(const BfsArgs& a, int ph, int32_t depth_next, uint32_t* next_bm, bool count_edges,
+                         LevelShared& s) {
     const uint32_t n_in = phase_input_len(a, ph);
     phase_begin(a, ph, n_in);
     const int32_t* qin = a.qk[ph & 1];
     const GlobalQueue out = out_queue(a, ph);
-    bool found = false;
-    unsigned long long eacc = 0;
+    TdSink sink{a, s, out, next_bm, depth_next, count_edges, false, 0ull};
     for (uint32_t base = blockIdx.x * BFS_THREADS; base < n_in; base += gridDim.x * BFS_THREADS) {
         const uint32_t i = base + threadIdx.x;
         int32_t u = -1, start = 0, deg = 0;
@@ -119,22 +197,11 @@ __device__ void td_phase(const BfsArgs& a, int ph, int32_t depth_next, LevelShar
                 deg = a.offs[u + 1] - start;
             }
         }
-        auto sink = [&](bool valid, int32_t v, int32_t src) {
-            const bool won = valid && claim_exact(a.visited, v);
-            if (won) {
-                a.depth[v] = depth_next;
-                a.pred[v] = src;
-                found = true;
-                if (COUNT_EDGES) eacc += (unsigned long long)(a.offs[v + 1] - a.offs[v]);
-            }
-            lq_append(s.qv, out, won, v, 0);
-        };
+        defer_heavy(a, ph, u, deg);
         tiered_gather(a.cols, u, start, deg, a.cta_th, a.wave_th, s.g, sink);
         lq_flush(s.qv, out, LQ_CAP / 2);
     }
-    lq_flush(s.qv, out, 1);
-    if (COUNT_EDGES) wg_add_u64(eacc, &a.ctr->level_edges[(ph + 1) & 3], s);
-    if (__syncthreads_or(found) && threadIdx.x == 0) atomic_max_u32(a.max_depth, (uint32_t)depth_next);
+    td_finish(a, ph, sink, s);
 }
 
 // Contract-expand (edge frontier -> edge frontier).  Incoming (vertex,
@@ -148,6 +215,7 @@ __device__ void ce_phase(const BfsArgs& a, int ph, int32_t depth_here, LevelShar
     const int32_t* qk = a.qk[ph & 1];
     const int32_t* qv = a.qv[ph & 1];
     const GlobalQueue out = out_queue(a, ph);
+    PairSink sink{s, out};
     bool found = false;
     for (uint32_t base = blockIdx.x * BFS_THREADS; base < n_in; base += gridDim.x * BFS_THREADS) {
         const uint32_t i = base + threadIdx.x;
@@ -167,7 +235,7 @@ __device__ void ce_phase(const BfsArgs& a, int ph, int32_t depth_here, LevelShar
                 deg = a.offs[v + 1] - start;
             }
         }
-        auto sink = [&](bool valid, int32_t nbr, int32_t src) { lq_append(s.qp, out, valid, nbr, src); };
+        defer_heavy(a, ph, v, deg);
         tiered_gather(a.cols, v, start, deg, a.cta_th, a.wave_th, s.g, sink);
         lq_flush(s.qp, out, LQ_CAP / 2);
     }
@@ -215,6 +283,7 @@ __device__ void tp_expand_phase(const BfsArgs& a, int ph, LevelShared& s) {
     phase_begin(a, ph, n_in);
     const int32_t* qin = a.qk[ph & 1];
     const GlobalQueue out = out_queue(a, ph);
+    PairSink sink{s, out};
     for (uint32_t base = blockIdx.x * BFS_THREADS; base < n_in; base += gridDim.x * BFS_THREADS) {
         const uint32_t i = base + threadIdx.x;
         int32_t u = -1, start = 0, deg = 0;
@@ -225,7 +294,7 @@ __device__ void tp_expand_phase(const BfsArgs& a, int ph, LevelShared& s) {
                 deg = a.offs[u + 1] - start;
             }
         }
-        auto sink = [&](bool valid, int32_t nbr, int32_t src) { lq_append(s.qp, out, valid, nbr, src); };
+        defer_heavy(a, ph, u, deg);
         tiered_gather(a.cols, u, start, deg, a.cta_th, a.wave_th, s.g, sink);
         lq_flush(s.qp, out, LQ_CAP / 2);
     }
@@ -262,7 +331,9 @@ __device__ void tp_filter_phase(const BfsArgs& a, int ph, LevelShared& s) {
 // current frontier bitmap and stops at the first hit.  A wave owns the two
 // bitmap words of its 64 vertices, so visited/next words are written without
 // atomics.  Discovered vertices are also appended to the vertex queue so a
-// top-down step can follow directly.
+// top-down step can follow directly.  Only valid on symmetric graphs.
+constexpr int32_t kBuWaveThreshold = 32;  // bottom-up: longer lists are scanned by the whole wave
+
 __device__ void bu_phase(const BfsArgs& a, int ph, int32_t depth_next, const uint32_t* cur, uint32_t* next,
                          LevelShared& s) {
     phase_begin(a, ph, 0);
@@ -278,16 +349,45 @@ __device__ void bu_phase(const BfsArgs& a, int ph, int32_t depth_next, const uin
         const uint32_t vw = (v < a.n) ? a.visited[v >> 5] : 0xFFFFFFFFu;
         const bool active = (v < a.n) && !((vw >> (v & 31)) & 1u);
         int32_t parent = -1;
-        int32_t deg = 0;
+        int32_t deg = 0, start = 0;
         if (active) {
-            const int32_t e0 = a.offs[v], e1 = a.offs[v + 1];
-            deg = e1 - e0;
-            for (int32_t e = e0; e < e1; ++e) {
-                const int32_t u = a.cols[e];
-                if ((cur[u >> 5] >> (u & 31)) & 1u) {
-                    parent = u;
+            start = a.offs[v];
+            deg = a.offs[v + 1] - start;
+        }
+        // Long lists: the whole wave scans one list 64 neighbours at a time and
+        // stops at the first chunk with a frontier neighbour (lowest lane wins,
+        // i.e. the first such neighbour in list order).
+        uint64_t lm = __ballot(active && deg > kBuWaveThreshold);
+        while (lm) {
+            const int leader = __ffsll((unsigned long long)lm) - 1;
+            lm &= lm - 1;
+            const int32_t ls = wave_bcast(start, leader), ld = wave_bcast(deg, leader);
+            int32_t hit_u = -1;
+            for (int32_t e0 = 0; e0 < ld; e0 += kWave) {
+                const int32_t e = e0 + lane;
+                const int32_t u = (e < ld) ? a.cols[ls + e] : -1;
+                const bool hit = u >= 0 && ((cur[u >> 5] >> (u & 31)) & 1u);
+                const uint64_t hm = __ballot(hit);
+                if (hm) {
+                    hit_u = wave_bcast(u, __ffsll((unsigned long long)hm) - 1);
                     break;
                 }
+            }
+            if (lane == leader) parent = hit_u;
+        }
+        // Short lists: each lane scans its own list, four neighbours per step.
+        if (active && deg <= kBuWaveThreshold) {
+            int32_t e = start;
+            const int32_t end = start + deg;
+            for (; e + 4 <= end && parent < 0; e += 4) {
+                const int32_t u0 = a.cols[e], u1 = a.cols[e + 1], u2 = a.cols[e + 2], u3 = a.cols[e + 3];
+                const uint32_t b0 = (cur[u0 >> 5] >> (u0 & 31)) & 1u, b1 = (cur[u1 >> 5] >> (u1 & 31)) & 1u;
+                const uint32_t b2 = (cur[u2 >> 5] >> (u2 & 31)) & 1u, b3 = (cur[u3 >> 5] >> (u3 & 31)) & 1u;
+                parent = b0 ? u0 : b1 ? u1 : b2 ? u2 : b3 ? u3 : -1;
+            }
+            for (; e < end && parent < 0; ++e) {
+                const int32_t u = a.cols[e];
+                if ((cur[u >> 5] >> (u & 31)) & 1u) parent = u;
             }
         }
         const bool found = parent >= 0;
@@ -313,15 +413,31 @@ __device__ void bu_phase(const BfsArgs& a, int ph, int32_t depth_next, const uin
     if (__syncthreads_or(found_any) && threadIdx.x == 0) atomic_max_u32(a.max_depth, (uint32_t)depth_next);
 }
 
+// Direction for level ph of the direction-optimizing traversal (Beamer,
+// Asanovic, Patterson, SC 2012).  Every workgroup evaluates the same counters
+// (finalized by the previous launch), so all agree without host involvement.
+// m_f = degree sum of the frontier, m_u = edges of unvisited vertices;
+// top-down -> bottom-up when m_f > m_u / alpha, back when n_f < n / beta.
+__device__ __forceinline__ uint32_t do_direction(const BfsArgs& a, int ph, uint32_t n_f) {
+    if (!a.symmetric) return 0;
+    const long long m_f = (long long)a.ctr->level_edges[ph & 3];
+    const long long vis = (long long)a.ctr->vis_edges[ph & 3];
+    const uint32_t prev = a.ctr->dir[(ph + 3) & 3];
+    long long m_u = a.edges - vis - m_f;
+    if (m_u < 0) m_u = 0;
+    if (prev == 0) return (m_f * (long long)a.do_alpha > m_u) ? 1u : 0u;
+    return ((long long)n_f * (long long)a.do_beta < (long long)a.n) ? 0u : 1u;
+}
+
 // ===========================================================================
 // Kernels.
 // ===========================================================================
 
-enum SeedMode { SEED_VERTEX = 0, SEED_EDGE = 1 };
+enum SeedMode { SEED_VERTEX = 0, SEED_EDGE = 1, SEED_DO = 2 };
 
 // Resets the counters and enqueues the source (the iteration-0 prologue of
 // contract_expand_atomic/kernel.cuh:307-332 and expand_contract_atomic/
-// kernel.cuh:305-319).  Vertex mode labels the source at once (depth 0,
+// kernel.cuh:305-319).  Vertex modes label the source at once (depth 0,
 // pred -2); edge mode enqueues (src, -2) for labelling at contraction.
 __global__ void k_seed(BfsArgs a, int ph, int32_t src, int mode) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -330,7 +446,11 @@ __global__ void k_seed(BfsArgs a, int ph, int32_t src, int mode) {
         c->qlen[i] = 0;
         c->level_edges[i] = 0;
         c->level_verts[i] = 0;
+        c->vis_edges[i] = 0;
+        c->dir[i] = 0;
     }
+    c->heavy_alloc[0] = 0;
+    c->heavy_alloc[1] = 0;
     c->overflow = 0;
     c->last_level = 0;
     c->levels_run = 0;
@@ -344,13 +464,14 @@ __global__ void k_seed(BfsArgs a, int ph, int32_t src, int mode) {
     c->corrupt = 0;
     *a.max_depth = 0;
     a.qk[ph & 1][0] = src;
-    if (mode == SEED_VERTEX) {
+    if (mode == SEED_EDGE) {
+        a.qv[ph & 1][0] = -2;
+    } else {
         a.visited[src >> 5] |= 1u << (src & 31);
         a.depth[src] = 0;
         a.pred[src] = -2;
         c->level_edges[ph & 3] = (unsigned long long)(a.offs[src + 1] - a.offs[src]);
-    } else {
-        a.qv[ph & 1][0] = -2;
+        if (mode == SEED_DO) a.fbm[ph % 3][src >> 5] |= 1u << (src & 31);
     }
     c->qlen[ph & 3] = 1;
 }
@@ -368,7 +489,7 @@ __device__ __forceinline__ void stats_end(unsigned long long t0, const BfsArgs& 
     }
 }
 
-enum PhaseKind { PK_TD = 0, PK_TD_COUNT = 1, PK_CE = 2, PK_TP_CONTRACT = 3, PK_TP_EXPAND = 4, PK_TP_FILTER = 5 };
+enum PhaseKind { PK_TD = 0, PK_CE = 2, PK_TP_CONTRACT = 3, PK_TP_EXPAND = 4, PK_TP_FILTER = 5 };
 
 template <int KIND>
 __global__ __launch_bounds__(BFS_THREADS) void k_phase(BfsArgs a, int ph, int32_t depth, int slot) {
@@ -376,8 +497,7 @@ __global__ __launch_bounds__(BFS_THREADS) void k_phase(BfsArgs a, int ph, int32_
     unsigned long long t0 = 0;
     stats_begin(t0, a);
     shared_init(s);
-    if (KIND == PK_TD) td_phase<false>(a, ph, depth, s);
-    if (KIND == PK_TD_COUNT) td_phase<true>(a, ph, depth, s);
+    if (KIND == PK_TD) td_phase(a, ph, depth, nullptr, false, s);
     if (KIND == PK_CE) ce_phase(a, ph, depth, s);
     if (KIND == PK_TP_CONTRACT) tp_contract_phase(a, ph, depth, s);
     if (KIND == PK_TP_EXPAND) tp_expand_phase(a, ph, s);
@@ -385,24 +505,91 @@ __global__ __launch_bounds__(BFS_THREADS) void k_phase(BfsArgs a, int ph, int32_
     stats_end(t0, a, slot);
 }
 
-__global__ __launch_bounds__(BFS_THREADS) void k_bottom_up(BfsArgs a, int ph, int32_t depth_next, int cur_sel,
-                                                           int slot) {
+// One level of the direction-optimizing traversal: the workgroups agree on
+// the direction from device counters, clear the bitmap two levels ahead and
+// run a top-down or bottom-up step.  An empty frontier makes it a no-op, so
+// the host can launch levels ahead of the termination check.
+__global__ __launch_bounds__(BFS_THREADS) void k_do_level(BfsArgs a, int ph, int slot) {
     __shared__ LevelShared s;
+    const uint32_t n_f = phase_input_len(a, ph);
+    if (n_f == 0) {
+        // Keep the slot rotation going so later no-op levels also read 0.
+        phase_begin(a, ph, 0);
+        return;
+    }
     unsigned long long t0 = 0;
     stats_begin(t0, a);
+    const uint32_t dir = do_direction(a, ph, n_f);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.ctr->dir[ph & 3] = dir;
+        a.ctr->vis_edges[(ph + 1) & 3] = a.ctr->vis_edges[ph & 3] + a.ctr->level_edges[ph & 3];
+    }
+    uint32_t* clear = a.fbm[(ph + 2) % 3];
+    for (int32_t w = blockIdx.x * BFS_THREADS + threadIdx.x; w < a.words; w += gridDim.x * BFS_THREADS) clear[w] = 0;
     shared_init(s);
-    bu_phase(a, ph, depth_next, a.fbm[cur_sel], a.fbm[cur_sel ^ 1], s);
+    if (dir == 0)
+        td_phase(a, ph, ph + 1, a.fbm[(ph + 1) % 3], true, s);
+    else
+        bu_phase(a, ph, ph + 1, a.fbm[ph % 3], a.fbm[(ph + 1) % 3], s);
     stats_end(t0, a, slot);
 }
 
-// Frontier queue (slot ph) -> frontier bitmap (top-down to bottom-up switch).
-__global__ void k_queue_to_bitmap(BfsArgs a, int ph, uint32_t* bm) {
-    const uint32_t n_in = ld_relaxed(&a.ctr->qlen[ph & 3]);
-    const int32_t* q = a.qk[ph & 1];
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n_in; i += gridDim.x * blockDim.x) {
-        const int32_t v = q[i];
-        if (valid_vertex(a, v)) atomic_or_u32(&bm[v >> 5], 1u << (v & 31));
+// Grid tier: the adjacency lists deferred by phase ph, cut into
+// HEAVY_CHUNK-edge chunks; workgroups take chunks grid-stride and find each
+// chunk's vertex through the chunk map.  Every chunk is checked against the
+// entry it names, so stale map words can never index outside a list.
+enum HeavyKind { HK_TD = 0, HK_TD_DO = 1, HK_PAIRS = 2 };
+
+template <int KIND>
+__global__ __launch_bounds__(BFS_THREADS) void k_heavy(BfsArgs a, int ph, int32_t depth_next, int slot) {
+    __shared__ LevelShared s;
+    const unsigned long long alloc =
+        __hip_atomic_load(&a.ctr->heavy_alloc[ph & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t H = min((uint32_t)(alloc >> 32), a.heavy_cap);
+    const uint32_t C = min((uint32_t)alloc, a.chunk_cap);
+    if (H == 0 || C == 0) return;
+    unsigned long long t0 = 0;
+    stats_begin(t0, a);
+    shared_init(s);
+    const GlobalQueue out = out_queue(a, ph);
+    TdSink tds{a, s, out, (KIND == HK_TD_DO) ? a.fbm[(ph + 1) % 3] : nullptr, depth_next, KIND == HK_TD_DO,
+               false, 0ull};
+    PairSink ps{s, out};
+    int it = 0;
+    for (uint32_t c = blockIdx.x; c < C; c += gridDim.x) {
+        const int32_t entry = a.chunk_map[c];
+        int32_t u = -1, start = 0, len = 0;
+        if ((uint32_t)entry < H) {
+            const int2 eu = a.heavy[entry];
+            if (valid_vertex(a, eu.x) && (uint32_t)eu.y <= c) {
+                const int32_t s0 = a.offs[eu.x], deg = a.offs[eu.x + 1] - s0;
+                const long long off = (long long)(c - (uint32_t)eu.y) * HEAVY_CHUNK;
+                if (off < deg) {
+                    u = eu.x;
+                    start = s0 + (int32_t)off;
+                    len = min((int32_t)HEAVY_CHUNK, deg - (int32_t)off);
+                }
+            }
+        }
+        const int32_t k = (int32_t)threadIdx.x;
+        const bool valid = k < len;
+        const int32_t v = valid ? a.cols[start + k] : 0;
+        if (KIND == HK_PAIRS)
+            ps(valid, v, u);
+        else
+            tds(valid, v, u);
+        if ((++it & 3) == 0) {
+            if (KIND == HK_PAIRS)
+                lq_flush(s.qp, out, LQ_CAP / 2);
+            else
+                lq_flush(s.qv, out, LQ_CAP / 2);
+        }
     }
+    if (KIND == HK_PAIRS)
+        lq_flush(s.qp, out, 1);
+    else
+        td_finish(a, ph, tds, s);
+    stats_end(t0, a, slot);
 }
 
 // Persistent kernels: all levels in one cooperative launch with a software
@@ -410,7 +597,8 @@ __global__ void k_queue_to_bitmap(BfsArgs a, int ph, uint32_t* bm) {
 // kernel.cuh:281-554; two_phase::Kernel, two_phase/kernel.cuh:60-313).
 // Every workgroup reads the same queue length after each barrier, so all
 // leave the loop together.  With `sat_limit` set, the kernel quits when a
-// frontier exceeds it (SATURATION_QUIT) and records where it stopped.
+// frontier exceeds it (SATURATION_QUIT) and records where it stopped.  The
+// grid tier is not used here (callers pass grid_th = INT_MAX).
 enum FusedKind { FK_TD = 0, FK_CE = 1, FK_TP = 2 };
 
 template <int KIND>
@@ -433,7 +621,7 @@ __global__ __launch_bounds__(BFS_THREADS) void k_fused(BfsArgs a, int ph, int32_
         unsigned long long tb = 0;
         if (a.wg_stats && threadIdx.x == 0) tb = wg_clock();
         if (KIND == FK_TD) {
-            td_phase<false>(a, ph, depth + 1, s);
+            td_phase(a, ph, depth + 1, nullptr, false, s);
             ph += 1;
         } else if (KIND == FK_CE) {
             ce_phase(a, ph, depth, s);
@@ -502,708 +690,4 @@ __global__ __launch_bounds__(BFS_THREADS) void k_gbarrier_check(uint32_t* bar, u
 
 }  // namespace b40c
 
-// ===========================================================================
-// Host objects.
-// ===========================================================================
-
-struct b40c_problem {
-    b40c_graph* g = nullptr;
-    hipStream_t stream = nullptr;
-    int32_t flags = 0;
-    b40c::DeviceBuffer depth, pred, visited, fbm0, fbm1, counters, maxdepth;
-    b40c::DeviceBuffer qk[2], qv[2];
-    size_t qcap[2] = {0, 0};
-    size_t vcap[2] = {0, 0};
-    int32_t frontier_type = -1;
-    bool ready = false;
-    b40c::DevCounters* h_ctr = nullptr;  // pinned readback
-};
-
-struct b40c_enactor {
-    int32_t strategy = B40C_HYBRID;
-    bool instrument = false;
-    b40c_tuning tuning{};
-    int64_t total_queued = 0;
-    int32_t search_depth = 0;
-    double avg_duty = 0.0;
-    b40c::DeviceBuffer wg_stats;
-    int launch_slots = 0;
-    int fused_grid_cache = 0;
-};
-
-namespace b40c {
-namespace {
-
-constexpr int kMaxLaunchSlots = 4096;
-
-// The persistent (cooperative, grid-barrier) kernels are switched off: a GPU
-// memory fault occurred in a run that exercised them and its cause has not
-// been shown yet.  Fused modes return B40C_ERR_UNSUPPORTED; the default modes
-// use one launch per level instead.
-constexpr bool kFusedEnabled = false;
-
-b40c_tuning default_tuning(const DeviceInfo& info) {
-    b40c_tuning t{};
-    t.grid_size = info.num_cus * 4;
-    t.cta_gather_threshold = 4096;
-    t.wave_gather_threshold = 64;
-    t.levels_per_sync = 4;
-    t.saturation_quit = 512;
-    t.do_alpha = 14;
-    t.do_beta = 24;
-    t.persistent_wgs_per_cu = 2;
-    return t;
-}
-
-void merge_tuning(b40c_tuning& dst, const b40c_tuning& src) {
-    if (src.grid_size > 0) dst.grid_size = src.grid_size;
-    if (src.cta_gather_threshold > 0) dst.cta_gather_threshold = src.cta_gather_threshold;
-    if (src.wave_gather_threshold > 0) dst.wave_gather_threshold = src.wave_gather_threshold;
-    if (src.levels_per_sync > 0) dst.levels_per_sync = src.levels_per_sync;
-    if (src.saturation_quit > 0) dst.saturation_quit = src.saturation_quit;
-    if (src.do_alpha > 0) dst.do_alpha = src.do_alpha;
-    if (src.do_beta > 0) dst.do_beta = src.do_beta;
-    if (src.persistent_wgs_per_cu > 0) dst.persistent_wgs_per_cu = src.persistent_wgs_per_cu;
-}
-
-int32_t frontier_type_of(int32_t strategy) {
-    switch (strategy) {
-        case B40C_EXPAND_CONTRACT:
-        case B40C_PERSISTENT:
-        case B40C_DIRECTION_OPTIMIZING:
-            return B40C_VERTEX_FRONTIERS;
-        case B40C_TWO_PHASE:
-            return B40C_MIXED_FRONTIERS;
-        default:
-            return B40C_EDGE_FRONTIERS;
-    }
-}
-
-BfsArgs make_args(b40c_problem* p, const b40c_enactor* e) {
-    BfsArgs a{};
-    a.offs = p->g->d_offsets();
-    a.cols = p->g->d_cols();
-    a.depth = p->depth.as<int32_t>();
-    a.pred = p->pred.as<int32_t>();
-    a.visited = p->visited.as<uint32_t>();
-    a.fbm[0] = p->fbm0.as<uint32_t>();
-    a.fbm[1] = p->fbm1.as<uint32_t>();
-    for (int i = 0; i < 2; ++i) {
-        a.qk[i] = p->qk[i].as<int32_t>();
-        a.qv[i] = p->qv[i].as<int32_t>();
-        a.qcap[i] = (uint32_t)std::min<size_t>(p->qcap[i], 0xFFFFFFFFull);
-    }
-    a.ctr = p->counters.as<DevCounters>();
-    a.max_depth = p->maxdepth.as<uint32_t>();
-    a.wg_stats = e->instrument ? e->wg_stats.as<unsigned long long>() : nullptr;
-    a.n = p->g->nodes;
-    a.words = (p->g->nodes + 31) / 32;
-    // Clamped so the fine tier's workgroup scan (< 256 x threshold) fits int32.
-    a.cta_th = std::min(std::max(1, e->tuning.cta_gather_threshold), 1 << 20);
-    a.wave_th = std::min(std::max(1, e->tuning.wave_gather_threshold), 1 << 20);
-    return a;
-}
-
-b40c_status read_counters(b40c_problem* p) {
-    B40C_HIP(hipMemcpyAsync(p->h_ctr, p->counters.ptr, sizeof(DevCounters), hipMemcpyDeviceToHost, p->stream));
-    B40C_HIP(hipStreamSynchronize(p->stream));
-    return B40C_OK;
-}
-
-struct Launcher {
-    b40c_problem* p;
-    b40c_enactor* e;
-    BfsArgs a;
-    unsigned grid;
-
-    int next_slot() {
-        if (!e->instrument) return 0;
-        int s = e->launch_slots;
-        if (s < kMaxLaunchSlots - 1) e->launch_slots++;
-        return s;
-    }
-    b40c_status phase(int kind, int ph, int32_t depth) {
-        const int slot = next_slot();
-        switch (kind) {
-            case PK_TD:
-                hipLaunchKernelGGL(k_phase<PK_TD>, dim3(grid), dim3(BFS_THREADS), 0, p->stream, a, ph, depth, slot);
-                break;
-            case PK_TD_COUNT:
-                hipLaunchKernelGGL(k_phase<PK_TD_COUNT>, dim3(grid), dim3(BFS_THREADS), 0, p->stream, a, ph, depth,
-                                   slot);
-                break;
-            case PK_CE:
-                hipLaunchKernelGGL(k_phase<PK_CE>, dim3(grid), dim3(BFS_THREADS), 0, p->stream, a, ph, depth, slot);
-                break;
-            case PK_TP_CONTRACT:
-                hipLaunchKernelGGL(k_phase<PK_TP_CONTRACT>, dim3(grid), dim3(BFS_THREADS), 0, p->stream, a, ph, depth,
-                                   slot);
-                break;
-            case PK_TP_EXPAND:
-                hipLaunchKernelGGL(k_phase<PK_TP_EXPAND>, dim3(grid), dim3(BFS_THREADS), 0, p->stream, a, ph, depth,
-                                   slot);
-                break;
-            case PK_TP_FILTER:
-                hipLaunchKernelGGL(k_phase<PK_TP_FILTER>, dim3(grid), dim3(BFS_THREADS), 0, p->stream, a, ph, depth,
-                                   slot);
-                break;
-        }
-        B40C_HIP(hipGetLastError());
-        return B40C_OK;
-    }
-    b40c_status seed(int ph, int32_t src, int mode) {
-        hipLaunchKernelGGL(k_seed, dim3(1), dim3(64), 0, p->stream, a, ph, src, mode);
-        B40C_HIP(hipGetLastError());
-        return B40C_OK;
-    }
-    b40c_status fused(int kind, int ph, int32_t depth, int max_levels, uint32_t sat, unsigned fgrid) {
-        if (!kFusedEnabled)
-            return fail(B40C_ERR_UNSUPPORTED, "persistent (fused) kernels are disabled in this build");
-        void* args[] = {&a, &ph, &depth, &max_levels, &sat};
-        const void* fn = nullptr;
-        if (kind == FK_TD) fn = (const void*)k_fused<FK_TD>;
-        if (kind == FK_CE) fn = (const void*)k_fused<FK_CE>;
-        if (kind == FK_TP) fn = (const void*)k_fused<FK_TP>;
-        B40C_HIP(hipLaunchCooperativeKernel(fn, dim3(fgrid), dim3(BFS_THREADS), args, 0, p->stream));
-        return B40C_OK;
-    }
-};
-
-// Largest co-resident grid for a fused kernel, capped by the tuning.
-b40c_status fused_grid(b40c_enactor* e, int kind, int max_grid, unsigned& out) {
-    DeviceInfo info;
-    B40C_TRY(device_info(info));
-    int per_cu = 0;
-    const void* fn = (kind == FK_TD) ? (const void*)k_fused<FK_TD>
-                     : (kind == FK_CE) ? (const void*)k_fused<FK_CE>
-                                       : (const void*)k_fused<FK_TP>;
-    B40C_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, BFS_THREADS, 0));
-    if (per_cu < 1) return fail(B40C_ERR_UNSUPPORTED, "fused kernel cannot be resident");
-    per_cu = std::min(per_cu, std::max(1, e->tuning.persistent_wgs_per_cu));
-    int g = per_cu * info.num_cus;
-    if (max_grid > 0) g = std::min(g, max_grid);
-    out = (unsigned)std::max(1, g);
-    return B40C_OK;
-}
-
-// ---- strategies --------------------------------------------------------------
-
-// Host level loop: launches `k` phases at a time, then checks the queue
-// length once (device-side termination makes surplus launches no-ops).
-template <typename StepFn>
-b40c_status level_loop(b40c_problem* p, b40c_enactor* e, int& ph, StepFn step, int phases_per_level) {
-    int k = std::max(1, e->tuning.levels_per_sync);
-    while (true) {
-        for (int j = 0; j < k; ++j) B40C_TRY(step());
-        B40C_TRY(read_counters(p));
-        if (p->h_ctr->overflow) break;
-        if (p->h_ctr->qlen[ph & 3] == 0) break;
-        k = std::min(k * 2, 64);
-    }
-    (void)phases_per_level;
-    return B40C_OK;
-}
-
-b40c_status enact_expand_contract(Launcher& L, int32_t src, bool fused) {
-    int ph = 0;
-    B40C_TRY(L.seed(ph, src, SEED_VERTEX));
-    if (fused) {
-        unsigned fg = 0;
-        B40C_TRY(fused_grid(L.e, FK_TD, 0, fg));
-        if (L.grid < fg) fg = L.grid;
-        return L.fused(FK_TD, ph, 0, 1 << 30, 0, fg);
-    }
-    int32_t depth = 0;
-    return level_loop(
-        L.p, L.e, ph,
-        [&]() -> b40c_status {
-            B40C_TRY(L.phase(PK_TD, ph, depth + 1));
-            ph += 1;
-            depth += 1;
-            return B40C_OK;
-        },
-        1);
-}
-
-b40c_status enact_contract_expand(Launcher& L, int32_t src, bool fused) {
-    int ph = 0;
-    B40C_TRY(L.seed(ph, src, SEED_EDGE));
-    if (fused) {
-        unsigned fg = 0;
-        B40C_TRY(fused_grid(L.e, FK_CE, 0, fg));
-        if (L.grid < fg) fg = L.grid;
-        return L.fused(FK_CE, ph, 0, 1 << 30, 0, fg);
-    }
-    int32_t depth = 0;
-    return level_loop(
-        L.p, L.e, ph,
-        [&]() -> b40c_status {
-            B40C_TRY(L.phase(PK_CE, ph, depth));
-            ph += 1;
-            depth += 1;
-            return B40C_OK;
-        },
-        1);
-}
-
-b40c_status enact_two_phase(Launcher& L, int32_t src, bool fused) {
-    // Mixed frontiers: buffer 1 holds edge frontiers, buffer 0 vertex
-    // frontiers, so contraction runs on odd phases.
-    int ph = 1;
-    B40C_TRY(L.seed(ph, src, SEED_EDGE));
-    if (fused) {
-        unsigned fg = 0;
-        B40C_TRY(fused_grid(L.e, FK_TP, 0, fg));
-        if (L.grid < fg) fg = L.grid;
-        return L.fused(FK_TP, ph, 0, 1 << 30, 0, fg);
-    }
-    int32_t depth = 0;
-    return level_loop(
-        L.p, L.e, ph,
-        [&]() -> b40c_status {
-            B40C_TRY(L.phase(PK_TP_CONTRACT, ph, depth));
-            B40C_TRY(L.phase(PK_TP_EXPAND, ph + 1, depth));
-            ph += 2;
-            depth += 1;
-            return B40C_OK;
-        },
-        2);
-}
-
-// Hybrid (enactor_hybrid.cuh:273-551): fused contract-expand while the edge
-// frontier stays below grid * saturation_quit; when it saturates, the host
-// runs filter -> contract -> expand steps until the filtered frontier is
-// small again, then resumes the fused kernel.
-b40c_status enact_hybrid(Launcher& L, int32_t src) {
-    b40c_problem* p = L.p;
-    b40c_enactor* e = L.e;
-    int ph = 0;
-    int32_t depth = 0;
-    B40C_TRY(L.seed(ph, src, SEED_EDGE));
-    unsigned fg = L.grid;
-    if (kFusedEnabled) {
-        B40C_TRY(fused_grid(e, FK_CE, 0, fg));
-        if (L.grid < fg) fg = L.grid;
-    }
-    const uint32_t sat = (uint32_t)std::max<int64_t>(1, (int64_t)fg * std::max(1, e->tuning.saturation_quit));
-    while (true) {
-        if (kFusedEnabled) {
-            B40C_TRY(L.fused(FK_CE, ph, depth, 1 << 30, sat, fg));
-            B40C_TRY(read_counters(p));
-            const uint32_t levels = p->h_ctr->levels_run;
-            ph = (int)p->h_ctr->last_level;
-            depth += (int32_t)levels;
-            if (p->h_ctr->overflow || !p->h_ctr->saturated) break;
-        } else {
-            // Narrow levels: one contract-expand launch per level.
-            bool saturated = false;
-            while (true) {
-                B40C_TRY(L.phase(PK_CE, ph, depth));
-                ph += 1;
-                depth += 1;
-                B40C_TRY(read_counters(p));
-                const uint32_t n = p->h_ctr->qlen[ph & 3];
-                if (p->h_ctr->overflow || n == 0) break;
-                if (n > sat) {
-                    saturated = true;
-                    break;
-                }
-            }
-            if (!saturated) break;
-        }
-        // Wide levels: filter -> contract -> expand, one launch each.
-        while (true) {
-            B40C_TRY(L.phase(PK_TP_FILTER, ph, depth));
-            B40C_TRY(L.phase(PK_TP_CONTRACT, ph + 1, depth));
-            B40C_TRY(L.phase(PK_TP_EXPAND, ph + 2, depth));
-            ph += 3;
-            depth += 1;
-            B40C_TRY(read_counters(p));
-            if (p->h_ctr->overflow) break;
-            const uint32_t n = p->h_ctr->qlen[ph & 3];
-            if (n == 0 || n <= sat) break;
-        }
-        if (p->h_ctr->overflow || p->h_ctr->qlen[ph & 3] == 0) break;
-    }
-    return B40C_OK;
-}
-
-// Direction-optimizing traversal (Beamer, Asanovic, Patterson, SC 2012): top
-// down while the frontier's edges are few, bottom up while the frontier is a
-// large share of the graph.  m_f = degree sum of the frontier, m_u = edges of
-// unvisited vertices; switch down->up when m_f > m_u / alpha, up->down when
-// n_f < n / beta.
-b40c_status enact_direction_optimizing(Launcher& L, int32_t src) {
-    b40c_problem* p = L.p;
-    b40c_enactor* e = L.e;
-    const int64_t n = p->g->nodes;
-    const int64_t E = p->g->edges;
-    const int alpha = std::max(1, e->tuning.do_alpha);
-    const int beta = std::max(1, e->tuning.do_beta);
-    int ph = 0;
-    int32_t depth = 0;
-    B40C_TRY(L.seed(ph, src, SEED_VERTEX));
-    int64_t m_visited = 0;  // degree sum of vertices visited so far (excluding frontier)
-    int64_t m_f = 0, n_f = 1;
-    B40C_TRY(read_counters(p));
-    m_f = (int64_t)p->h_ctr->level_edges[ph & 3];
-    bool bottom_up = false;
-    int cur_sel = 0;
-    while (n_f > 0) {
-        const int64_t m_u = E - m_visited - m_f;
-        if (!bottom_up && m_f > m_u / alpha) {
-            bottom_up = true;
-            cur_sel = 0;
-            B40C_HIP(hipMemsetAsync(p->fbm0.ptr, 0, p->fbm0.bytes, p->stream));
-            hipLaunchKernelGGL(k_queue_to_bitmap, dim3(L.grid), dim3(256), 0, p->stream, L.a, ph,
-                               p->fbm0.as<uint32_t>());
-            B40C_HIP(hipGetLastError());
-        } else if (bottom_up && n_f < n / beta) {
-            bottom_up = false;
-        }
-        m_visited += m_f;
-        if (bottom_up) {
-            const int slot = L.next_slot();
-            hipLaunchKernelGGL(k_bottom_up, dim3(L.grid), dim3(BFS_THREADS), 0, p->stream, L.a, ph, depth + 1,
-                               cur_sel, slot);
-            B40C_HIP(hipGetLastError());
-            cur_sel ^= 1;
-        } else {
-            B40C_TRY(L.phase(PK_TD_COUNT, ph, depth + 1));
-        }
-        ph += 1;
-        depth += 1;
-        B40C_TRY(read_counters(p));
-        if (p->h_ctr->overflow) break;
-        n_f = p->h_ctr->qlen[ph & 3];
-        m_f = (int64_t)p->h_ctr->level_edges[ph & 3];
-    }
-    return B40C_OK;
-}
-
-}  // namespace
-}  // namespace b40c
-
-using namespace b40c;
-
-extern "C" {
-
-b40c_status b40c_problem_create_from_graph(b40c_graph* g, int32_t flags, b40c_problem** out) {
-    if (!g || !out) return fail(B40C_ERR_INVALID_ARG, "b40c_problem_create_from_graph: null argument");
-    if (flags != 0) return fail(B40C_ERR_INVALID_ARG, "b40c_problem_create_from_graph: unknown flags %d", flags);
-    b40c_problem* p = new b40c_problem();
-    p->flags = flags;
-    hipError_t err = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking);
-    if (err != hipSuccess) {
-        delete p;
-        return fail(B40C_ERR_HIP, "hipStreamCreate failed: %s", hipGetErrorString(err));
-    }
-    err = hipHostMalloc((void**)&p->h_ctr, sizeof(DevCounters), hipHostMallocDefault);
-    if (err != hipSuccess) {
-        (void)hipStreamDestroy(p->stream);
-        delete p;
-        return fail(B40C_ERR_HIP, "hipHostMalloc failed: %s", hipGetErrorString(err));
-    }
-    graph_retain(g);
-    p->g = g;
-    const size_t n = (size_t)g->nodes;
-    const size_t words = (n + 31) / 32;
-    b40c_status s = p->depth.alloc(n * 4);
-    if (s == B40C_OK) s = p->pred.alloc(n * 4);
-    if (s == B40C_OK) s = p->visited.alloc(words * 4);
-    if (s == B40C_OK) s = p->fbm0.alloc(words * 4);
-    if (s == B40C_OK) s = p->fbm1.alloc(words * 4);
-    if (s == B40C_OK) s = p->counters.alloc(sizeof(DevCounters));
-    if (s == B40C_OK) s = p->maxdepth.alloc(256);
-    if (s != B40C_OK) {
-        b40c_problem_destroy(p);
-        return s;
-    }
-    *out = p;
-    return B40C_OK;
-}
-
-b40c_status b40c_problem_create(int32_t nodes, int64_t edges, const int32_t* column_indices,
-                                const int32_t* row_offsets, int32_t flags, b40c_problem** out) {
-    b40c_graph* g = nullptr;
-    B40C_TRY(b40c_graph_from_host_csr(nodes, edges, row_offsets, column_indices, &g));
-    b40c_status s = b40c_problem_create_from_graph(g, flags, out);
-    graph_release(g);  // the problem holds its own reference
-    return s;
-}
-
-b40c_status b40c_problem_reset(b40c_problem* p, int32_t frontier_type, double queue_sizing) {
-    if (!p) return fail(B40C_ERR_INVALID_ARG, "b40c_problem_reset: null problem");
-    if (frontier_type < B40C_VERTEX_FRONTIERS || frontier_type > B40C_MIXED_FRONTIERS)
-        return fail(B40C_ERR_INVALID_ARG, "b40c_problem_reset: bad frontier type %d", frontier_type);
-    if (queue_sizing <= 0.0) queue_sizing = 1.3;
-    const double n = (double)p->g->nodes, m = (double)std::max<int64_t>(p->g->edges, 1);
-    // Queue sizes per frontier type (csr_problem.cuh:602-641).
-    size_t want[2], wantv[2];
-    if (frontier_type == B40C_VERTEX_FRONTIERS) {
-        want[0] = want[1] = (size_t)(n * queue_sizing);
-        wantv[0] = wantv[1] = 0;
-    } else if (frontier_type == B40C_EDGE_FRONTIERS) {
-        want[0] = want[1] = (size_t)(m * queue_sizing);
-        wantv[0] = wantv[1] = want[0];
-    } else {
-        want[0] = (size_t)(n * queue_sizing);
-        want[1] = (size_t)(m * queue_sizing);
-        wantv[0] = 0;
-        wantv[1] = want[1];
-    }
-    for (int i = 0; i < 2; ++i) {
-        want[i] = std::max<size_t>(want[i], 1);
-        want[i] = std::min<size_t>(want[i], 0xFFFFFFF0ull);
-        wantv[i] = std::min<size_t>(wantv[i], 0xFFFFFFF0ull);
-        if (p->qk[i].bytes < want[i] * 4) B40C_TRY(p->qk[i].alloc(want[i] * 4));
-        if (wantv[i] && p->qv[i].bytes < wantv[i] * 4) B40C_TRY(p->qv[i].alloc(wantv[i] * 4));
-        // Capacity is what this reset asked for, so a small queue_sizing
-        // really limits the frontier (and can overflow).
-        p->qcap[i] = want[i];
-        p->vcap[i] = wantv[i];
-    }
-    p->frontier_type = frontier_type;
-    const size_t nb = (size_t)p->g->nodes * 4;
-    B40C_HIP(hipMemsetAsync(p->depth.ptr, 0xFF, nb, p->stream));
-    B40C_HIP(hipMemsetAsync(p->pred.ptr, 0xFF, nb, p->stream));
-    B40C_HIP(hipMemsetAsync(p->visited.ptr, 0, p->visited.bytes, p->stream));
-    p->ready = true;
-    return B40C_OK;
-}
-
-b40c_status b40c_problem_extract(b40c_problem* p, int32_t* depth, int32_t* pred) {
-    if (!p) return fail(B40C_ERR_INVALID_ARG, "b40c_problem_extract: null problem");
-    const size_t nb = (size_t)p->g->nodes * 4;
-    if (depth) B40C_HIP(hipMemcpyAsync(depth, p->depth.ptr, nb, hipMemcpyDeviceToHost, p->stream));
-    if (pred) B40C_HIP(hipMemcpyAsync(pred, p->pred.ptr, nb, hipMemcpyDeviceToHost, p->stream));
-    B40C_HIP(hipStreamSynchronize(p->stream));
-    return B40C_OK;
-}
-
-void* b40c_problem_stream(b40c_problem* p) { return p ? (void*)p->stream : nullptr; }
-
-b40c_status b40c_problem_device_labels(b40c_problem* p, int32_t** d_depth, int32_t** d_pred) {
-    if (!p) return fail(B40C_ERR_INVALID_ARG, "b40c_problem_device_labels: null problem");
-    if (d_depth) *d_depth = p->depth.as<int32_t>();
-    if (d_pred) *d_pred = p->pred.as<int32_t>();
-    return B40C_OK;
-}
-
-void b40c_problem_destroy(b40c_problem* p) {
-    if (!p) return;
-    if (p->stream) (void)hipStreamSynchronize(p->stream);
-    p->depth.release();
-    p->pred.release();
-    p->visited.release();
-    p->fbm0.release();
-    p->fbm1.release();
-    p->counters.release();
-    p->maxdepth.release();
-    for (int i = 0; i < 2; ++i) {
-        p->qk[i].release();
-        p->qv[i].release();
-    }
-    if (p->h_ctr) (void)hipHostFree(p->h_ctr);
-    if (p->stream) (void)hipStreamDestroy(p->stream);
-    graph_release(p->g);
-    delete p;
-}
-
-b40c_status b40c_enactor_create(int32_t strategy, int32_t instrument, b40c_enactor** out) {
-    if (!out) return fail(B40C_ERR_INVALID_ARG, "b40c_enactor_create: null out");
-    if (strategy < B40C_EXPAND_CONTRACT || strategy > B40C_DIRECTION_OPTIMIZING)
-        return fail(B40C_ERR_INVALID_ARG, "b40c_enactor_create: unknown strategy %d", strategy);
-    DeviceInfo info;
-    B40C_TRY(device_info(info));
-    if (info.arch.rfind("gfx950", 0) != 0)
-        return fail(B40C_ERR_UNSUPPORTED, "b40c_enactor_create: device %s is not gfx950", info.arch.c_str());
-    b40c_enactor* e = new b40c_enactor();
-    e->strategy = strategy;
-    e->instrument = instrument != 0;
-    e->tuning = default_tuning(info);
-    if (e->instrument) {
-        b40c_status s = e->wg_stats.alloc((2 + 2 * kMaxLaunchSlots) * sizeof(unsigned long long));
-        if (s != B40C_OK) {
-            delete e;
-            return s;
-        }
-    }
-    *out = e;
-    return B40C_OK;
-}
-
-b40c_status b40c_enactor_set_tuning(b40c_enactor* e, const b40c_tuning* t) {
-    if (!e || !t) return fail(B40C_ERR_INVALID_ARG, "b40c_enactor_set_tuning: null argument");
-    merge_tuning(e->tuning, *t);
-    return B40C_OK;
-}
-
-b40c_status b40c_enactor_get_tuning(const b40c_enactor* e, b40c_tuning* t) {
-    if (!e || !t) return fail(B40C_ERR_INVALID_ARG, "b40c_enactor_get_tuning: null argument");
-    *t = e->tuning;
-    return B40C_OK;
-}
-
-int32_t b40c_enactor_frontier_type(const b40c_enactor* e) { return e ? frontier_type_of(e->strategy) : -1; }
-
-b40c_status b40c_enact_search(b40c_enactor* e, b40c_problem* p, int32_t src, int32_t max_grid_size, int32_t mode) {
-    if (!e || !p) return fail(B40C_ERR_INVALID_ARG, "b40c_enact_search: null argument");
-    if (!p->ready) return fail(B40C_ERR_INVALID_ARG, "b40c_enact_search: problem not reset since last search");
-    if (src < 0 || src >= p->g->nodes)
-        return fail(B40C_ERR_INVALID_ARG, "b40c_enact_search: source %d out of range [0,%d)", src, p->g->nodes);
-    const int32_t need = frontier_type_of(e->strategy);
-    const bool edge_ok = (p->frontier_type == B40C_EDGE_FRONTIERS);
-    if (p->frontier_type != need && !(need == B40C_VERTEX_FRONTIERS && edge_ok) &&
-        !(need == B40C_MIXED_FRONTIERS && edge_ok))
-        return fail(B40C_ERR_INVALID_ARG, "b40c_enact_search: problem reset with frontier type %d, strategy needs %d",
-                    p->frontier_type, need);
-    if (mode < B40C_MODE_DEFAULT || mode > B40C_MODE_ITERATIVE)
-        return fail(B40C_ERR_INVALID_ARG, "b40c_enact_search: bad mode %d", mode);
-    p->ready = false;
-
-    Launcher L{p, e, make_args(p, e), 0};
-    int g = e->tuning.grid_size;
-    if (max_grid_size > 0) g = std::min(g, max_grid_size);
-    L.grid = (unsigned)std::max(1, g);
-    e->launch_slots = 0;
-    if (e->instrument) {
-        std::vector<unsigned long long> init(2 + 2 * kMaxLaunchSlots);
-        init[0] = init[1] = 0;
-        for (int i = 0; i < kMaxLaunchSlots; ++i) {
-            init[2 + 2 * i] = ~0ull;
-            init[3 + 2 * i] = 0;
-        }
-        B40C_HIP(hipMemcpyAsync(e->wg_stats.ptr, init.data(), init.size() * 8, hipMemcpyHostToDevice, p->stream));
-        B40C_HIP(hipStreamSynchronize(p->stream));
-    }
-
-    b40c_status s = B40C_OK;
-    switch (e->strategy) {
-        case B40C_EXPAND_CONTRACT:
-            // Fused is the reference's default form for this strategy (test_bfs.cu:690-695).
-            s = enact_expand_contract(L, src, mode == B40C_MODE_FUSED || (mode == B40C_MODE_DEFAULT && kFusedEnabled));
-            break;
-        case B40C_PERSISTENT:
-            s = enact_expand_contract(L, src, mode != B40C_MODE_ITERATIVE);
-            break;
-        case B40C_CONTRACT_EXPAND:
-            s = enact_contract_expand(L, src, mode == B40C_MODE_FUSED || (mode == B40C_MODE_DEFAULT && kFusedEnabled));
-            break;
-        case B40C_TWO_PHASE:
-            s = enact_two_phase(L, src, mode == B40C_MODE_FUSED);
-            break;
-        case B40C_HYBRID:
-            s = (mode == B40C_MODE_DEFAULT) ? enact_hybrid(L, src)
-                                            : enact_contract_expand(L, src, mode == B40C_MODE_FUSED);
-            break;
-        case B40C_DIRECTION_OPTIMIZING:
-            s = enact_direction_optimizing(L, src);
-            break;
-    }
-    if (s != B40C_OK) return s;
-    B40C_TRY(read_counters(p));
-    uint32_t maxd = 0;
-    B40C_HIP(hipMemcpy(&maxd, p->maxdepth.ptr, sizeof(uint32_t), hipMemcpyDeviceToHost));
-    e->total_queued = (int64_t)p->h_ctr->queued;
-    e->search_depth = (int32_t)maxd + 1;
-    e->avg_duty = 0.0;
-    if (e->instrument) {
-        std::vector<unsigned long long> st(2 + 2 * kMaxLaunchSlots);
-        B40C_HIP(hipMemcpy(st.data(), e->wg_stats.ptr, st.size() * 8, hipMemcpyDeviceToHost));
-        double life = (double)st[1];
-        if (life == 0.0) {
-            // Per-level kernels: lifetime = launch span x workgroups.
-            for (int i = 0; i < e->launch_slots; ++i)
-                if (st[3 + 2 * i] > st[2 + 2 * i]) life += (double)(st[3 + 2 * i] - st[2 + 2 * i]) * L.grid;
-        }
-        e->avg_duty = life > 0 ? (double)st[0] / life : 0.0;
-    }
-    if (p->h_ctr->bar_abort) return fail(B40C_ERR_HIP, "grid barrier wait timed out (grid not co-resident?)");
-    if (p->h_ctr->corrupt) return fail(B40C_ERR_HIP, "frontier queue held a vertex id outside [0, n)");
-    if (p->h_ctr->overflow)
-        return fail(B40C_ERR_QUEUE_OVERFLOW, "Frontier queue overflow. Please increase queue-sizing factor.");
-    return B40C_OK;
-}
-
-b40c_status b40c_get_statistics(b40c_enactor* e, int64_t* total_queued, int32_t* search_depth, double* avg_duty) {
-    if (!e) return fail(B40C_ERR_INVALID_ARG, "b40c_get_statistics: null enactor");
-    if (total_queued) *total_queued = e->total_queued;
-    if (search_depth) *search_depth = e->search_depth;
-    if (avg_duty) *avg_duty = e->avg_duty;
-    return B40C_OK;
-}
-
-void b40c_enactor_destroy(b40c_enactor* e) { delete e; }
-
-b40c_status b40c_gbarrier_benchmark(int32_t grid_size, int32_t iterations, double* ms_per_sync) {
-    if (grid_size < 1 || iterations < 1 || !ms_per_sync)
-        return fail(B40C_ERR_INVALID_ARG, "b40c_gbarrier_benchmark: bad arguments");
-    DeviceInfo info;
-    B40C_TRY(device_info(info));
-    int per_cu = 0;
-    B40C_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_gbarrier_bench, BFS_THREADS, 0));
-    if (grid_size > per_cu * info.num_cus)
-        return fail(B40C_ERR_INVALID_ARG, "b40c_gbarrier_benchmark: grid %d exceeds co-resident limit %d", grid_size,
-                    per_cu * info.num_cus);
-    DeviceBuffer bar;
-    B40C_TRY(bar.alloc(256));
-    B40C_HIP(hipMemset(bar.ptr, 0, 256));
-    hipEvent_t e0, e1;
-    B40C_HIP(hipEventCreate(&e0));
-    B40C_HIP(hipEventCreate(&e1));
-    uint32_t* barp = bar.as<uint32_t>();
-    int iters = iterations;
-    void* args[] = {&barp, &iters};
-    B40C_HIP(hipEventRecord(e0, nullptr));
-    B40C_HIP(hipLaunchCooperativeKernel((const void*)k_gbarrier_bench, dim3(grid_size), dim3(BFS_THREADS), args, 0,
-                                        nullptr));
-    B40C_HIP(hipEventRecord(e1, nullptr));
-    B40C_HIP(hipEventSynchronize(e1));
-    uint32_t aborted = 0;
-    B40C_HIP(hipMemcpy(&aborted, barp + 2, sizeof(uint32_t), hipMemcpyDeviceToHost));
-    if (aborted) return fail(B40C_ERR_HIP, "grid barrier wait timed out");
-    float ms = 0;
-    B40C_HIP(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    *ms_per_sync = ms / iterations;
-    return B40C_OK;
-}
-
-b40c_status b40c_gbarrier_check(int32_t grid_size, int32_t iterations, int64_t* mismatches) {
-    if (grid_size < 2 || iterations < 1 || !mismatches)
-        return fail(B40C_ERR_INVALID_ARG, "b40c_gbarrier_check: bad arguments");
-    DeviceInfo info;
-    B40C_TRY(device_info(info));
-    int per_cu = 0;
-    B40C_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_gbarrier_check, BFS_THREADS, 0));
-    if (grid_size > per_cu * info.num_cus)
-        return fail(B40C_ERR_INVALID_ARG, "b40c_gbarrier_check: grid %d exceeds co-resident limit %d", grid_size,
-                    per_cu * info.num_cus);
-    const int words = 1024;
-    DeviceBuffer bar, data, bad;
-    B40C_TRY(bar.alloc(256));
-    B40C_TRY(data.alloc((size_t)grid_size * words * sizeof(uint32_t)));
-    B40C_TRY(bad.alloc(sizeof(unsigned long long)));
-    B40C_HIP(hipMemset(bar.ptr, 0, 256));
-    B40C_HIP(hipMemset(data.ptr, 0, data.bytes));
-    B40C_HIP(hipMemset(bad.ptr, 0, sizeof(unsigned long long)));
-    uint32_t* barp = bar.as<uint32_t>();
-    uint32_t* datap = data.as<uint32_t>();
-    unsigned long long* badp = bad.as<unsigned long long>();
-    int w = words, iters = iterations;
-    void* args[] = {&barp, &datap, &w, &iters, &badp};
-    B40C_HIP(hipLaunchCooperativeKernel((const void*)k_gbarrier_check, dim3(grid_size), dim3(BFS_THREADS), args, 0,
-                                        nullptr));
-    B40C_HIP(hipDeviceSynchronize());
-    uint32_t aborted = 0;
-    unsigned long long h_bad = 0;
-    B40C_HIP(hipMemcpy(&aborted, barp + 2, sizeof(uint32_t), hipMemcpyDeviceToHost));
-    B40C_HIP(hipMemcpy(&h_bad, badp, sizeof(h_bad), hipMemcpyDeviceToHost));
-    if (aborted) return fail(B40C_ERR_HIP, "grid barrier wait timed out");
-    *mismatches = (int64_t)h_bad;
-    return B40C_OK;
-}
-
-}  // extern "C"
+#include "bfs_host.inc"

@@ -37,7 +37,11 @@ struct BfsArgs {
     int32_t* depth;
     int32_t* pred;
     uint32_t* visited;       // 1 bit per vertex
-    uint32_t* fbm[2];        // frontier bitmaps (bottom-up)
+    uint32_t* fbm[3];        // frontier bitmaps (direction-optimizing), rotating by level % 3
+    int2* heavy;             // grid-tier entries (vertex, first chunk), in chunk order
+    int32_t* chunk_map;      // grid-tier chunk -> entry index
+    uint32_t heavy_cap;      // capacity of heavy[] (entries)
+    uint32_t chunk_cap;      // capacity of chunk_map[]
     int32_t* qk[2];          // frontier queue keys (ping-pong)
     int32_t* qv[2];          // frontier queue values (predecessors), edge frontiers only
     uint32_t qcap[2];        // capacities of qk[i] (elements)
@@ -48,7 +52,14 @@ struct BfsArgs {
     int32_t words;           // ceil(n / 32)
     int32_t cta_th;
     int32_t wave_th;
+    int32_t grid_th;         // degree >= this: grid tier (split across workgroups)
+    int32_t symmetric;       // graph has every reverse edge (bottom-up allowed)
+    int32_t do_alpha;
+    int32_t do_beta;
+    long long edges;
 };
+
+constexpr int HEAVY_CHUNK = 256;     // grid-tier edges per work item (one per thread)
 
 // ---------------------------------------------------------------------------
 // LDS-buffered queue.

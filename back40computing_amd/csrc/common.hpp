@@ -56,6 +56,9 @@ struct alignas(256) DevCounters {
     uint32_t corrupt;            // set when a frontier held an id outside [0, n)
     unsigned long long level_edges[4];  // rotating per-level degree sums (direction switch)
     uint32_t level_verts[4];            // rotating per-level discovered-vertex counts
+    unsigned long long vis_edges[4];    // degree sum of vertices visited before the level
+    uint32_t dir[4];                    // direction chosen per level (0 top-down, 1 bottom-up)
+    unsigned long long heavy_alloc[2];  // grid tier by level parity: (entries << 32) | chunks
 };
 
 // ---------------------------------------------------------------------------

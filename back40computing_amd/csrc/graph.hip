@@ -38,6 +38,30 @@ void graph_release(b40c_graph* g) {
 }
 
 namespace {
+__global__ void k_check_symmetric(const int32_t* offs, const int32_t* cols, int32_t n, uint32_t* flags);
+}
+
+b40c_status graph_symmetric(b40c_graph* g, bool& symmetric) {
+    int known = g->symmetric.load();
+    if (known < 0) {
+        DeviceBuffer flags;
+        B40C_TRY(flags.alloc(2 * sizeof(uint32_t)));
+        B40C_HIP(hipMemset(flags.ptr, 0, 2 * sizeof(uint32_t)));
+        if (g->edges > 0) {
+            hipLaunchKernelGGL(k_check_symmetric, dim3(4096), dim3(256), 0, nullptr, g->d_offsets(), g->d_cols(),
+                               g->nodes, flags.as<uint32_t>());
+            B40C_HIP(hipGetLastError());
+        }
+        uint32_t h[2] = {0, 0};
+        B40C_HIP(hipMemcpy(h, flags.ptr, sizeof(h), hipMemcpyDeviceToHost));
+        known = (h[0] == 0 && h[1] == 0) ? 1 : 0;
+        g->symmetric.store(known);
+    }
+    symmetric = known == 1;
+    return B40C_OK;
+}
+
+namespace {
 
 constexpr int GB_THREADS = 256;
 
@@ -177,6 +201,40 @@ __global__ __launch_bounds__(GB_THREADS) void k_row_offsets(const uint32_t* rows
     }
 }
 
+// Symmetry check used before bottom-up steps: one wave per vertex u; lanes
+// stride u's columns, check that they are strictly increasing and that each
+// neighbour v lists u (binary search in v's sorted columns).  flags[0] counts
+// unsorted rows or out-of-range ids, flags[1] missing reverse edges.
+__global__ __launch_bounds__(GB_THREADS) void k_check_symmetric(const int32_t* offs, const int32_t* cols, int32_t n,
+                                                                uint32_t* flags) {
+    const int lane = threadIdx.x % kWave;
+    const int64_t wave0 = ((int64_t)blockIdx.x * GB_THREADS + threadIdx.x) / kWave;
+    const int64_t nwaves = (int64_t)gridDim.x * GB_THREADS / kWave;
+    uint32_t bad_order = 0, missing = 0;
+    for (int64_t u = wave0; u < n; u += nwaves) {
+        const int32_t s = offs[u], e = offs[u + 1];
+        for (int32_t k = s + lane; k < e; k += kWave) {
+            const int32_t v = cols[k];
+            if (v < 0 || v >= n) {
+                ++bad_order;
+                continue;
+            }
+            if (k + 1 < e && cols[k + 1] <= v) ++bad_order;
+            int32_t lo = offs[v], hi = offs[v + 1];
+            while (lo < hi) {
+                const int32_t mid = lo + (hi - lo) / 2;
+                if (cols[mid] < (int32_t)u)
+                    lo = mid + 1;
+                else
+                    hi = mid;
+            }
+            if (lo >= offs[v + 1] || cols[lo] != (int32_t)u) ++missing;
+        }
+    }
+    if (bad_order) atomic_add_u32(&flags[0], bad_order);
+    if (missing) atomic_add_u32(&flags[1], missing);
+}
+
 __global__ void k_u32_to_i32_copy(const uint32_t* src, int32_t* dst, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         dst[i] = (int32_t)src[i];
@@ -250,7 +308,9 @@ b40c_status build_rmat(int scale, int edge_factor, double a, double b, double c,
     int32_t sel = 0;
     B40C_TRY(radix_sort_u64(k0.as<uint64_t>(), k1.as<uint64_t>(), nkeys, 0, 2 * scale, &sel, stream));
     const uint64_t* sorted = sel ? k1.as<uint64_t>() : k0.as<uint64_t>();
-    return csr_from_sorted_keys(sorted, nkeys, scale, sentinel, n, g, stream);
+    B40C_TRY(csr_from_sorted_keys(sorted, nkeys, scale, sentinel, n, g, stream));
+    g->symmetric.store(1);  // both directions of every sampled edge were emitted
+    return B40C_OK;
 }
 
 b40c_status build_lattice(int w, uint64_t seed, int32_t* h_perm, b40c_graph* g) {
@@ -279,6 +339,7 @@ b40c_status build_lattice(int w, uint64_t seed, int32_t* h_perm, b40c_graph* g) 
     B40C_TRY(csr_from_sorted_keys(sel ? k1.as<uint64_t>() : k0.as<uint64_t>(), nkeys, bits, sentinel, (int32_t)n, g,
                                   stream));
     if (h_perm) B40C_HIP(hipMemcpy(h_perm, perm.ptr, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost));
+    g->symmetric.store(1);  // every lattice edge is emitted from both endpoints
     return B40C_OK;
 }
 

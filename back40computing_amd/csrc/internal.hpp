@@ -53,6 +53,9 @@ struct b40c_graph {
     b40c::DeviceBuffer offsets;  // int32[nodes + 1]
     b40c::DeviceBuffer cols;     // int32[edges]
     std::atomic<int> refs{1};
+    // 1: every edge (u,v) has its reverse (v,u); 0: not (or not provable);
+    // -1: not yet checked.  Bottom-up BFS steps are only valid when 1.
+    std::atomic<int> symmetric{-1};
     const int32_t* d_offsets() const { return offsets.as<int32_t>(); }
     const int32_t* d_cols() const { return cols.as<int32_t>(); }
 };
@@ -60,4 +63,6 @@ struct b40c_graph {
 namespace b40c {
 void graph_retain(b40c_graph* g);
 void graph_release(b40c_graph* g);
+// Resolves g->symmetric (runs a GPU check the first time it is unknown).
+b40c_status graph_symmetric(b40c_graph* g, bool& symmetric);
 }  // namespace b40c

@@ -169,7 +169,12 @@ def main():
         for s in (0, 1, 2, 3, 4, 5):
             if s == args.strategy:
                 continue
-            r = measure_strategy(bfs, b4, hip, graph, srcs, e_trav, v_vis, s, args.compare_steps, 2)
+            try:
+                r = measure_strategy(bfs, b4, hip, graph, srcs, e_trav, v_vis, s, args.compare_steps, 2)
+            except b4.B40CError as ex:
+                per_strategy[names[s]] = None
+                log("strategy %s: unavailable (%s)" % (names[s], ex))
+                continue
             per_strategy[names[s]] = round(r["gteps"], 3)
             log("strategy %s: %.3f GTEPS (%.3f ms/search)" % (names[s], r["gteps"], r["ms_per_search_enact"]))
 

@@ -75,6 +75,7 @@ typedef struct b40c_tuning {
     int32_t do_alpha;              /* direction-optimizing: switch to bottom-up when m_f > m_u / alpha */
     int32_t do_beta;               /* direction-optimizing: back to top-down when n_f < n / beta   */
     int32_t persistent_wgs_per_cu; /* persistent kernels: resident workgroups per CU        */
+    int32_t grid_gather_threshold; /* degree >= this: list split over all workgroups (grid tier) */
 } b40c_tuning;
 
 /* ---- errors / device --------------------------------------------------- */

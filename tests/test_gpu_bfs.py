@@ -123,7 +123,9 @@ def test_gather_tiers(bfsmod, orc, sm):
     edges += [(int(a), int(b)) for a, b in rng.integers(1, n, (30000, 2))]
     off, cols = csr_from_edges(n, edges, undirected=True)
     g = bfsmod.Graph.from_host_csr(off, cols)
-    for tuning in (None, {"cta_gather_threshold": 300, "wave_gather_threshold": 8}):
+    for tuning in (None, {"cta_gather_threshold": 300, "wave_gather_threshold": 8},
+                   {"grid_gather_threshold": 50, "cta_gather_threshold": 20, "wave_gather_threshold": 4},
+                   {"grid_gather_threshold": 1 << 30, "cta_gather_threshold": 1 << 30}):
         for s in (0, 9050, 15000):
             depth, pred, st = run(bfsmod, g, s, *sm, tuning=tuning)
             check(orc, off, cols, s, depth, pred, st)
