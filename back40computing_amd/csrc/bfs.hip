@@ -451,6 +451,7 @@ __global__ void k_seed(BfsArgs a, int ph, int32_t src, int mode) {
     }
     c->heavy_alloc[0] = 0;
     c->heavy_alloc[1] = 0;
+    c->ph_base = 0;
     c->overflow = 0;
     c->last_level = 0;
     c->levels_run = 0;
@@ -489,11 +490,26 @@ __device__ __forceinline__ void stats_end(unsigned long long t0, const BfsArgs& 
     }
 }
 
+// In a captured, replayed level loop (dev_base set) a kernel's phase
+// argument is relative to ctr->ph_base, which the loop's last node advances,
+// and its depth argument is relative to that phase.
+__device__ __forceinline__ void resolve_phase(const BfsArgs& a, int& ph, int32_t& depth) {
+    if (a.dev_base) {
+        ph += (int)ld_relaxed(&a.ctr->ph_base);
+        depth += ph;
+    }
+}
+
+__global__ void k_advance(BfsArgs a, int steps) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) a.ctr->ph_base += (uint32_t)steps;
+}
+
 enum PhaseKind { PK_TD = 0, PK_CE = 2, PK_TP_CONTRACT = 3, PK_TP_EXPAND = 4, PK_TP_FILTER = 5 };
 
 template <int KIND>
 __global__ __launch_bounds__(BFS_THREADS) void k_phase(BfsArgs a, int ph, int32_t depth, int slot) {
     __shared__ LevelShared s;
+    resolve_phase(a, ph, depth);
     unsigned long long t0 = 0;
     stats_begin(t0, a);
     shared_init(s);
@@ -511,6 +527,8 @@ __global__ __launch_bounds__(BFS_THREADS) void k_phase(BfsArgs a, int ph, int32_
 // the host can launch levels ahead of the termination check.
 __global__ __launch_bounds__(BFS_THREADS) void k_do_level(BfsArgs a, int ph, int slot) {
     __shared__ LevelShared s;
+    int32_t unused_depth = 0;
+    resolve_phase(a, ph, unused_depth);
     const uint32_t n_f = phase_input_len(a, ph);
     if (n_f == 0) {
         // Keep the slot rotation going so later no-op levels also read 0.
@@ -543,6 +561,7 @@ enum HeavyKind { HK_TD = 0, HK_TD_DO = 1, HK_PAIRS = 2 };
 template <int KIND>
 __global__ __launch_bounds__(BFS_THREADS) void k_heavy(BfsArgs a, int ph, int32_t depth_next, int slot) {
     __shared__ LevelShared s;
+    resolve_phase(a, ph, depth_next);
     const unsigned long long alloc =
         __hip_atomic_load(&a.ctr->heavy_alloc[ph & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const uint32_t H = min((uint32_t)(alloc >> 32), a.heavy_cap);

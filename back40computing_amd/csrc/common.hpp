@@ -58,6 +58,7 @@ struct alignas(256) DevCounters {
     uint32_t level_verts[4];            // rotating per-level discovered-vertex counts
     unsigned long long vis_edges[4];    // degree sum of vertices visited before the level
     uint32_t dir[4];                    // direction chosen per level (0 top-down, 1 bottom-up)
+    uint32_t ph_base;                   // graph-replayed level loops: phase of the replay's first step
     unsigned long long heavy_alloc[2];  // grid tier by level parity: (entries << 32) | chunks
 };
 

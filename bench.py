@@ -61,10 +61,18 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def measure_strategy(bfs, b4, hip, graph, srcs, e_trav, v_vis, strategy, steps, warmup, mode=0):
+def tuning_for(strategy, graph, arch):
+    """Autotuned knobs from the committed tuning table (None: library defaults)."""
+    from back40computing_amd import autotune
+    return autotune.lookup(arch, strategy, graph.nodes, graph.edges)
+
+
+def measure_strategy(bfs, b4, hip, graph, srcs, e_trav, v_vis, strategy, steps, warmup, mode=0, tuning=None):
     prob = bfs.CsrProblem()
     b4._lib.check(prob.FromGraph(graph))
     en = bfs.ENACTORS[strategy]()
+    if tuning:
+        en.set_tuning(**tuning)
     ft = en.GetFrontierType()
     stream = prob.stream()
     lib = b4.lib()
@@ -153,13 +161,16 @@ def main():
     # ---- untimed pass: reached vertices / traversed edges per source ------
     e_trav, v_vis = [], []
     for s in srcs:
-        depth, pred, st = bfs.bfs(graph, int(s), strategy=args.strategy)
+        depth, pred, st = bfs.bfs(graph, int(s), strategy=args.strategy,
+                                  tuning=tuning_for(args.strategy, graph, info["arch"]))
         m = depth >= 0
         e_trav.append(int(deg[m].sum()))
         v_vis.append(int(m.sum()))
     log("per-source reached: mean V_vis=%.0f mean E_trav=%.0f" % (np.mean(v_vis), np.mean(e_trav)))
 
-    head = measure_strategy(bfs, b4, hip, graph, srcs, e_trav, v_vis, args.strategy, args.steps, args.warmup)
+    head_tuning = tuning_for(args.strategy, graph, info["arch"])
+    head = measure_strategy(bfs, b4, hip, graph, srcs, e_trav, v_vis, args.strategy, args.steps, args.warmup,
+                            tuning=head_tuning)
     log("headline strategy %d: %s" % (args.strategy, json.dumps(head)))
 
     names = {0: "expand_contract", 1: "contract_expand", 2: "two_phase", 3: "hybrid", 4: "persistent",
@@ -170,7 +181,8 @@ def main():
             if s == args.strategy:
                 continue
             try:
-                r = measure_strategy(bfs, b4, hip, graph, srcs, e_trav, v_vis, s, args.compare_steps, 2)
+                r = measure_strategy(bfs, b4, hip, graph, srcs, e_trav, v_vis, s, args.compare_steps, 2,
+                                     tuning=tuning_for(s, graph, info["arch"]))
             except b4.B40CError as ex:
                 per_strategy[names[s]] = None
                 log("strategy %s: unavailable (%s)" % (names[s], ex))
@@ -212,6 +224,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": "rmat%d_ef16_a0.57_b0.19_c0.19_sym_dedup_single_source_bfs" % args.scale,
                    "vertices": n, "csr_edges": E, "sources": len(srcs), "strategy": names[args.strategy],
+                   "tuning": head_tuning or "library defaults",
                    "step": "Reset + EnactSearch, depth + predecessor output"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
