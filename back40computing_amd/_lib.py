@@ -59,6 +59,7 @@ class Tuning(ctypes.Structure):
         ("do_beta", ctypes.c_int32),
         ("persistent_wgs_per_cu", ctypes.c_int32),
         ("grid_gather_threshold", ctypes.c_int32),
+        ("bottom_up_wave_threshold", ctypes.c_int32),
     ]
 
     def as_dict(self):
@@ -108,6 +109,18 @@ SIGNATURES = {
     "b40c_enactor_destroy": (None, [_vp]),
     "b40c_gbarrier_benchmark": (_i32, [_i32, _i32, _p_d]),
     "b40c_gbarrier_check": (_i32, [_i32, _i32, _p_i64]),
+    "b40c_multigraph_pack": (_i32, [_vp, _i32, _pp]),
+    "b40c_multigraph_info": (_i32, [_vp, _p_i32, _vp, _vp]),
+    "b40c_multigraph_destroy": (None, [_vp]),
+    "b40c_msbfs_create": (_i32, [_vp, _i64, _i32, _pp]),
+    "b40c_msbfs_set_tuning": (_i32, [_vp, _i32, _i32, _i32]),
+    "b40c_msbfs_submit": (_i32, [_vp, _i32, _i32, _p_i32]),
+    "b40c_msbfs_step": (_i32, [_vp, _p_i32, _p_i32]),
+    "b40c_msbfs_run": (_i32, [_vp]),
+    "b40c_msbfs_poll": (_i32, [_vp, _vp, _i32, _p_i32]),
+    "b40c_msbfs_result": (_i32, [_vp, _i32, _vp, _vp, _p_i32]),
+    "b40c_msbfs_stream": (_vp, [_vp]),
+    "b40c_msbfs_destroy": (None, [_vp]),
 }
 
 _lib = None

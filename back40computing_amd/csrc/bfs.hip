@@ -332,8 +332,6 @@ __device__ void tp_filter_phase(const BfsArgs& a, int ph, LevelShared& s) {
 // bitmap words of its 64 vertices, so visited/next words are written without
 // atomics.  Discovered vertices are also appended to the vertex queue so a
 // top-down step can follow directly.  Only valid on symmetric graphs.
-constexpr int32_t kBuWaveThreshold = 32;  // bottom-up: longer lists are scanned by the whole wave
-
 __device__ void bu_phase(const BfsArgs& a, int ph, int32_t depth_next, const uint32_t* cur, uint32_t* next,
                          LevelShared& s) {
     phase_begin(a, ph, 0);
@@ -357,7 +355,7 @@ __device__ void bu_phase(const BfsArgs& a, int ph, int32_t depth_next, const uin
         // Long lists: the whole wave scans one list 64 neighbours at a time and
         // stops at the first chunk with a frontier neighbour (lowest lane wins,
         // i.e. the first such neighbour in list order).
-        uint64_t lm = __ballot(active && deg > kBuWaveThreshold);
+        uint64_t lm = __ballot(active && deg > a.bu_wave_th);
         while (lm) {
             const int leader = __ffsll((unsigned long long)lm) - 1;
             lm &= lm - 1;
@@ -376,7 +374,7 @@ __device__ void bu_phase(const BfsArgs& a, int ph, int32_t depth_next, const uin
             if (lane == leader) parent = hit_u;
         }
         // Short lists: each lane scans its own list, four neighbours per step.
-        if (active && deg <= kBuWaveThreshold) {
+        if (active && deg <= a.bu_wave_th) {
             int32_t e = start;
             const int32_t end = start + deg;
             for (; e + 4 <= end && parent < 0; e += 4) {
@@ -487,6 +485,23 @@ __device__ __forceinline__ void stats_end(unsigned long long t0, const BfsArgs& 
         // per-launch span: [2 + 2*slot] = min start, [3 + 2*slot] = max end
         __hip_atomic_fetch_min(&a.wg_stats[2 + 2 * slot], t0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_fetch_max(&a.wg_stats[3 + 2 * slot], t1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// Reset's label initialisation in one launch (MemsetKernel x 2 in the
+// reference, csr_problem.cuh:723-745): depth and predecessor := -1, visited
+// mask and the first two frontier bitmaps := 0.
+__global__ __launch_bounds__(256) void k_reset(int32_t* depth, int32_t* pred, uint32_t* visited, uint32_t* f0,
+                                               uint32_t* f1, int32_t n, int32_t words) {
+    const int32_t stride = gridDim.x * blockDim.x;
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        depth[i] = -1;
+        pred[i] = -1;
+    }
+    for (int32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < words; w += stride) {
+        visited[w] = 0;
+        f0[w] = 0;
+        f1[w] = 0;
     }
 }
 

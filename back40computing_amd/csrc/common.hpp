@@ -59,6 +59,7 @@ struct alignas(256) DevCounters {
     unsigned long long vis_edges[4];    // degree sum of vertices visited before the level
     uint32_t dir[4];                    // direction chosen per level (0 top-down, 1 bottom-up)
     uint32_t ph_base;                   // graph-replayed level loops: phase of the replay's first step
+    uint32_t max_depth;                 // deepest depth label written by the search
     unsigned long long heavy_alloc[2];  // grid tier by level parity: (entries << 32) | chunks
 };
 

@@ -120,6 +120,157 @@ def measure_strategy(bfs, b4, hip, graph, srcs, e_trav, v_vis, strategy, steps, 
     return res
 
 
+def pct(x, q):
+    return float(np.percentile(np.asarray(x), q)) if len(x) else None
+
+
+def bench_batched(b4, info):
+    """1,024 queries over 256 RMAT graphs (scale integer-uniform in [10,18],
+    4 sources each): the batched multi-source engine vs the plain
+    single-query enactor run query by query, labels compared for every query."""
+    from back40computing_amd import serve
+    from back40computing_amd.batch import MultiGraph, MultiSourceBFS
+    from back40computing_amd import autotune
+    t0 = time.perf_counter()
+    graphs, offsets, scales, queries = serve.batched_workload()
+    build_s = time.perf_counter() - t0
+    mg = MultiGraph(graphs)
+    log("batched workload: %d graphs, %d queries, V=%d E=%d (built in %.2f s)" % (
+        len(graphs), len(queries), int(mg.vbase[-1]), int(mg.ebase[-1]), build_s))
+    plain = serve.PlainServer(graphs, b4.DIRECTION_OPTIMIZING, tuning_for=lambda g: autotune.lookup(
+        info["arch"], b4.DIRECTION_OPTIMIZING, graphs[g].nodes, graphs[g].edges))
+    for g, s in queries[:16]:
+        plain.run_one(g, s)
+    plain_ms = []
+    t0 = time.perf_counter()
+    for g, s in queries:
+        t1 = time.perf_counter()
+        plain.run_one(g, s)
+        plain_ms.append((time.perf_counter() - t1) * 1e3)
+    plain_s = time.perf_counter() - t0
+    budget = int(sum(graphs[g].nodes for g, _ in queries) * 8 * 1.05) + (1 << 20)
+    eng = MultiSourceBFS(mg, budget, len(queries) + 8)
+    ids = {}
+    done_ms = {}
+    t0 = time.perf_counter()
+    for g, s in queries:
+        ids[eng.submit(g, s)] = (g, s)
+    levels = 0
+    while True:
+        r, w = eng.step()
+        levels += 1
+        now = (time.perf_counter() - t0) * 1e3
+        for q in eng.poll():
+            done_ms[q] = now
+        if r == 0 and w == 0:
+            break
+    batch_s = time.perf_counter() - t0
+    # labels: batched vs plain, every query
+    mismatches = 0
+    for q, (g, s) in ids.items():
+        d_b, _, _ = eng.result(q, g, want_pred=False)
+        d_p, _ = plain.run_one(g, s, want=True)
+        mismatches += int(not np.array_equal(d_b, d_p))
+    lat = list(done_ms.values())
+    res = {
+        "queries": len(queries), "graphs": len(graphs), "levels": levels,
+        "plain_qps": round(len(queries) / plain_s, 1), "batched_qps": round(len(queries) / batch_s, 1),
+        "speedup": round(plain_s / batch_s, 2),
+        "plain_ms_p50": round(pct(plain_ms, 50), 4), "plain_ms_p99": round(pct(plain_ms, 99), 4),
+        "batched_ms_p50": round(pct(lat, 50), 3), "batched_ms_p99": round(pct(lat, 99), 3),
+        "labels_identical": mismatches == 0, "label_mismatches": mismatches,
+        "note": "batched latency = batch start -> query finished (all 1,024 submitted at once); plain = one query "
+                "alone (Reset + EnactSearch, direction-optimizing)",
+    }
+    eng.close()
+    plain.close()
+    mg.close()
+    log("batched:", res)
+    return res
+
+
+def bench_served(b4, info, duration_s=1.5):
+    """Resident RMAT-23 + 512 RMAT graphs of scale 12-16; Poisson arrivals
+    (90 % on the big graph); continuous batching vs a FIFO plain server."""
+    from back40computing_amd import serve
+    from back40computing_amd.batch import MultiGraph
+    from back40computing_amd.bfs import Graph
+    from back40computing_amd import autotune
+    t0 = time.perf_counter()
+    big = Graph.rmat(23, 16, 0.57, 0.19, 0.19, 7)
+    pool_scales = [12 + int(serve._uniform(77, k) * 5) for k in range(512)]
+    pool, pool_off = serve.build_pool(pool_scales, seed0=50000)
+    graphs = [big] + pool
+    big_off = np.empty(big.nodes + 1, np.int32)
+    b4._lib.check(b4.lib().b40c_graph_copy_to_host(big.handle, b4._lib.ptr(big_off), None))
+    big_src = pick_sources_cached(big_off, 11, 4096)
+    pool_src = [pick_sources_cached(o, 100 + i, 8) for i, o in enumerate(pool_off)]
+    mg = MultiGraph(graphs)
+    log("served workload: V=%d E=%d (built in %.2f s)" % (int(mg.vbase[-1]), int(mg.ebase[-1]),
+                                                          time.perf_counter() - t0))
+
+    def choose(u, k):
+        if u < 0.9:
+            return (0, int(big_src[k % len(big_src)]))
+        g = 1 + int((u - 0.9) / 0.1 * 512) % 512
+        return (g, int(pool_src[g - 1][k % len(pool_src[g - 1])]))
+
+    tun = {0: autotune.lookup(info["arch"], b4.DIRECTION_OPTIMIZING, big.nodes, big.edges)}
+    plain = serve.PlainServer(graphs, b4.DIRECTION_OPTIMIZING, tuning_for=lambda g: tun.get(g))
+    probe = [choose(serve._uniform(5, k), k) for k in range(40)]
+    for g, s in probe[:8]:
+        plain.run_one(g, s)
+    t1 = time.perf_counter()
+    for g, s in probe:
+        plain.run_one(g, s)
+    service = (time.perf_counter() - t1) / len(probe)
+    cap = 1.0 / service
+    out = {"plain_service_ms": round(service * 1e3, 3), "plain_capacity_qps": round(cap, 1), "points": []}
+    rate_plain = 0.7 * cap
+    trace = serve.poisson_trace(rate_plain, duration_s, choose, seed=21)
+    log("served: plain service %.3f ms, capacity %.0f qps; plain trace of %d queries" % (
+        service * 1e3, cap, len(trace)))
+    lat, wall = plain.serve(trace, log=log)
+    out["points"].append({"server": "plain", "offered_qps": round(rate_plain, 1), "queries": len(trace),
+                          "achieved_qps": round(len(trace) / wall, 1), "p50_ms": round(pct(lat, 50) * 1e3, 3),
+                          "p99_ms": round(pct(lat, 99) * 1e3, 3)})
+    cont = serve.ContinuousServer(mg, label_budget_bytes=6 << 30, log=log)
+    for mult in (5, 10):
+        rate = mult * rate_plain
+        trace = serve.poisson_trace(rate, duration_s, choose, seed=21 + mult)
+        log("served: continuous trace of %d queries at %.0f qps" % (len(trace), rate))
+        lat, wall = cont.serve(trace)
+        out["points"].append({"server": "continuous", "offered_qps": round(rate, 1), "queries": len(trace),
+                              "achieved_qps": round(len(trace) / wall, 1), "p50_ms": round(pct(lat, 50) * 1e3, 3),
+                              "p99_ms": round(pct(lat, 99) * 1e3, 3)})
+    # Alone vs in the mix: labels of a sample of queries served both ways.
+    sample = [choose(serve._uniform(99, k), k) for k in range(24)]
+    ids = {cont.eng.submit(g, s): (g, s) for g, s in sample}
+    cont.eng.run()
+    same = 0
+    for q in cont.eng.poll():
+        g, s = ids[q]
+        d_mix, _, _ = cont.eng.result(q, g, want_pred=False)
+        d_alone, _ = plain.run_one(g, s, want=True)
+        same += int(np.array_equal(d_mix, d_alone))
+    out["labels_identical"] = same == len(sample)
+    p_plain = out["points"][0]
+    best = max((p for p in out["points"][1:] if p["p99_ms"] <= p_plain["p99_ms"]),
+               key=lambda p: p["achieved_qps"], default=None)
+    out["qps_ratio_at_equal_or_lower_p99"] = (round(best["achieved_qps"] / p_plain["achieved_qps"], 2)
+                                              if best else None)
+    cont.eng.close()
+    plain.close()
+    mg.close()
+    log("served:", out)
+    return out
+
+
+def pick_sources_cached(off, seed, count):
+    from back40computing_amd.sources import pick_sources
+    return pick_sources(off, seed, count)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -130,6 +281,8 @@ def main():
     ap.add_argument("--strategy", type=int, default=5, help="headline strategy (5 = direction-optimizing)")
     ap.add_argument("--compare-steps", type=int, default=16, help="steps for the other strategies (0: skip)")
     ap.add_argument("--cpu-sources", type=int, default=6, help="oracle BFS sources timed for cpu_baseline (0: skip)")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched 1,024-query workload")
+    ap.add_argument("--no-served", action="store_true", help="skip the served Poisson workload")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -190,6 +343,15 @@ def main():
             per_strategy[names[s]] = round(r["gteps"], 3)
             log("strategy %s: %.3f GTEPS (%.3f ms/search)" % (names[s], r["gteps"], r["ms_per_search_enact"]))
 
+    batched = None if args.no_batched else bench_batched(b4, info)
+    served = None
+    if not args.no_served:
+        try:
+            served = bench_served(b4, info)
+        except (RuntimeError, b4.B40CError) as ex:
+            served = {"error": str(ex)}
+            log("served workload failed:", ex)
+
     # ---- CPU baseline: the serial oracle BFS on a bounded sample ----------
     cpu = None
     if args.cpu_sources > 0:
@@ -229,6 +391,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
         "cpu_baseline": cpu,
+        "batched": batched,
+        "served": served,
         "detail": {"gteps_enact_only": round(head["gteps_enact_only"], 3),
                    "ms_per_search_enact": round(head["ms_per_search_enact"], 4),
                    "b_alg_per_search": head["b_alg_per_search"],

@@ -76,6 +76,7 @@ typedef struct b40c_tuning {
     int32_t do_beta;               /* direction-optimizing: back to top-down when n_f < n / beta   */
     int32_t persistent_wgs_per_cu; /* persistent kernels: resident workgroups per CU        */
     int32_t grid_gather_threshold; /* degree >= this: list split over all workgroups (grid tier) */
+    int32_t bottom_up_wave_threshold; /* bottom-up steps: longer lists are scanned by a whole wave */
 } b40c_tuning;
 
 /* ---- errors / device --------------------------------------------------- */
@@ -165,6 +166,43 @@ b40c_status b40c_enact_search(b40c_enactor* e, b40c_problem* p, int32_t src, int
 b40c_status b40c_get_statistics(b40c_enactor* e, int64_t* total_queued, int32_t* search_depth,
                                 double* avg_duty);
 void b40c_enactor_destroy(b40c_enactor* e);
+
+/* ---- batched and served traversal (no counterpart in the reference) ------
+ * Many single-source queries over several resident graphs share one
+ * bit-parallel multi-source traversal: up to 64 queries per graph run in the
+ * same level steps, each with its own int32 depth/predecessor label slot
+ * (local vertex ids).  Queries join at level boundaries as a graph slot and
+ * label memory (8 bytes x graph vertices, from the budget given at creation)
+ * become free, and leave when their traversal ends — continuous batching.
+ * Graphs must be symmetric (every edge present in both directions). */
+typedef struct b40c_multigraph b40c_multigraph;
+typedef struct b40c_msbfs b40c_msbfs;
+/* Packs device graphs into one CSR (graph g owns vertices
+ * [vbase[g], vbase[g+1])).  The source graphs may be destroyed afterwards. */
+b40c_status b40c_multigraph_pack(b40c_graph* const* graphs, int32_t ngraphs, b40c_multigraph** out);
+/* vbase: ngraphs + 1 entries, ebase: ngraphs + 1 entries (either nullable). */
+b40c_status b40c_multigraph_info(const b40c_multigraph* mg, int32_t* ngraphs, int32_t* vbase, int64_t* ebase);
+void b40c_multigraph_destroy(b40c_multigraph* mg);
+b40c_status b40c_msbfs_create(b40c_multigraph* mg, int64_t label_budget_bytes, int32_t max_queries,
+                              b40c_msbfs** out);
+/* Direction switch parameters and workgroups per level launch (0 keeps). */
+b40c_status b40c_msbfs_set_tuning(b40c_msbfs* m, int32_t alpha, int32_t beta, int32_t grid);
+/* Queues a query (source `src` local to graph `graph`); returns its id.
+ * B40C_ERR_ADMISSION if its labels can never fit the budget. */
+b40c_status b40c_msbfs_submit(b40c_msbfs* m, int32_t graph, int32_t src, int32_t* query_id);
+/* One level boundary: admit waiting queries, run one level of every query in
+ * flight, retire finished ones.  Reports queries running and waiting. */
+b40c_status b40c_msbfs_step(b40c_msbfs* m, int32_t* running, int32_t* waiting);
+/* Steps until every submitted query has finished. */
+b40c_status b40c_msbfs_run(b40c_msbfs* m);
+/* Finished queries not reported before (up to max_ids). */
+b40c_status b40c_msbfs_poll(b40c_msbfs* m, int32_t* ids, int32_t max_ids, int32_t* count);
+/* Labels of a finished query (nullable outputs) and its search depth; frees
+ * its label slot. */
+b40c_status b40c_msbfs_result(b40c_msbfs* m, int32_t query_id, int32_t* depth, int32_t* pred,
+                              int32_t* search_depth);
+void* b40c_msbfs_stream(b40c_msbfs* m);
+void b40c_msbfs_destroy(b40c_msbfs* m);
 
 /* ---- diagnostics ---------------------------------------------------------- */
 /* Times `iterations` back-to-back grid barriers of a cooperatively launched

@@ -1,0 +1,130 @@
+"""GPU parity of the batched / served multi-source engine.
+
+Every query's depth labels must equal the serial oracle's on its own graph,
+predecessors must pass the reference's invariants (test/bfs/test_bfs.cu:
+387-438), and a query's labels must be identical whether it runs alone or in
+a mix with others (continuous batching, slot reuse, memory admission)."""
+import numpy as np
+import pytest
+
+from helpers import csr_from_edges
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def mods():
+    from back40computing_amd import batch, bfs
+    return bfs, batch
+
+
+@pytest.fixture(scope="module")
+def pool(mods, orc):
+    """Six symmetric graphs: RMAT of several scales and a lattice."""
+    bfs, batch = mods
+    specs = [("rmat", 8, 1), ("rmat", 11, 2), ("rmat", 13, 3), ("lattice", 12, 4), ("rmat", 10, 5), ("rmat", 6, 6)]
+    graphs, host = [], []
+    for kind, p, seed in specs:
+        g = bfs.Graph.rmat(p, 16, 0.57, 0.19, 0.19, seed) if kind == "rmat" else bfs.Graph.lattice3d(p, seed)
+        graphs.append(g)
+        host.append(g.to_host())
+    mg = batch.MultiGraph(graphs)
+    return mg, host
+
+
+def check_query(orc, host, g, src, depth, pred, sd):
+    off, cols = host[g]
+    ref, _, rsd = orc.bfs(off, cols, int(src), with_pred=False)
+    np.testing.assert_array_equal(depth, ref)
+    assert orc.check_pred(off, cols, ref, pred) == -1
+    assert sd == rsd
+
+
+def run_all(eng, mg, queries):
+    ids = {}
+    for g, s in queries:
+        ids[eng.submit(g, s)] = (g, s)
+    eng.run()
+    done = eng.poll()
+    assert sorted(done) == sorted(ids)
+    return {q: (ids[q], eng.result(q, ids[q][0])) for q in done}
+
+
+def sources(orc, host, g, count, seed):
+    return [int(x) for x in orc.pick_sources(host[g][0], seed, count)]
+
+
+def test_batch_all_graphs(mods, orc, pool):
+    bfs, batch = mods
+    mg, host = pool
+    queries = [(g, s) for g in range(mg.ngraphs) for s in sources(orc, host, g, 4, 10 + g)]
+    eng = batch.MultiSourceBFS(mg, label_budget_bytes=1 << 28)
+    res = run_all(eng, mg, queries)
+    for q, ((g, s), (depth, pred, sd)) in res.items():
+        check_query(orc, host, g, s, depth, pred, sd)
+
+
+def test_alone_equals_in_mix(mods, orc, pool):
+    bfs, batch = mods
+    mg, host = pool
+    mix = [(g, s) for g in range(mg.ngraphs) for s in sources(orc, host, g, 6, 20 + g)]
+    eng = batch.MultiSourceBFS(mg, label_budget_bytes=1 << 28)
+    in_mix = {(g, s): r for q, ((g, s), r) in run_all(eng, mg, mix).items()}
+    for g, s in mix[::5]:
+        alone_eng = batch.MultiSourceBFS(mg, label_budget_bytes=1 << 28)
+        (_, (depth, pred, sd)), = run_all(alone_eng, mg, [(g, s)]).values()
+        md, mp, msd = in_mix[(g, s)]
+        np.testing.assert_array_equal(depth, md)
+        assert sd == msd
+        check_query(orc, host, g, s, depth, pred, sd)
+
+
+def test_continuous_batching_and_slot_reuse(mods, orc, pool):
+    """Queries arrive while others are in flight; more than 64 on one graph
+    forces slot reuse; a small label budget forces memory admission."""
+    bfs, batch = mods
+    mg, host = pool
+    g_big = 2
+    budget = 8 * (3 * mg.nodes(g_big))  # room for about three queries on the largest graph
+    eng = batch.MultiSourceBFS(mg, label_budget_bytes=budget)
+    waves = [[(g_big, s) for s in sources(orc, host, g_big, 5, 31)],
+             [(1, s) for s in sources(orc, host, 1, 70, 32)],
+             [(3, s) for s in sources(orc, host, 3, 6, 33)] + [(5, s) for s in sources(orc, host, 5, 3, 34)]]
+    ids = {}
+    finished = {}
+    for wave in waves:
+        for g, s in wave:
+            ids[eng.submit(g, s)] = (g, s)
+        for _ in range(2):
+            eng.step()
+        for q in eng.poll():
+            finished[q] = eng.result(q, ids[q][0])
+    while True:
+        r, w = eng.step()
+        for q in eng.poll():
+            finished[q] = eng.result(q, ids[q][0])
+        if r == 0 and w == 0:
+            break
+    assert sorted(finished) == sorted(ids)
+    for q, (depth, pred, sd) in finished.items():
+        g, s = ids[q]
+        check_query(orc, host, g, s, depth, pred, sd)
+
+
+def test_admission_errors(mods, orc, pool):
+    bfs, batch = mods
+    from back40computing_amd import B40CError, _lib
+    mg, host = pool
+    eng = batch.MultiSourceBFS(mg, label_budget_bytes=64)
+    with pytest.raises(B40CError) as ei:
+        eng.submit(2, 0)
+    assert ei.value.status == _lib.ERR_ADMISSION
+    with pytest.raises(B40CError):
+        eng.submit(0, mg.nodes(0))
+    # A directed graph is refused by the batched engine.
+    off, cols = csr_from_edges(4, [[0, 1], [1, 2], [2, 0], [2, 3]], undirected=False)
+    d = bfs.Graph.from_host_csr(off, cols)
+    dm = batch.MultiGraph([d])
+    with pytest.raises(B40CError) as ei:
+        batch.MultiSourceBFS(dm, label_budget_bytes=1 << 20)
+    assert ei.value.status == _lib.ERR_UNSUPPORTED
