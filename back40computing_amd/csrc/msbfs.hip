@@ -14,11 +14,23 @@
 //    indexed by local vertex id; depth = level + 1 - qstart[q]; pred is a
 //    local id (-2 for the source, -1 unreached).
 //
-// One launch per level: every workgroup derives the direction from device
-// counters (top-down push with 64-bit atomic ORs on sparse levels, bottom-up
-// pull with per-vertex early exit on dense ones), clears the mask array two
-// levels ahead, and records per-graph "new bits" so the host can retire
-// finished queries.
+// Direction is chosen per query, not per level: queries arrive at different
+// times, so one level mixes young queries (a handful of frontier vertices)
+// with mature ones (frontiers covering much of their graph).  Each level
+// runs these launches:
+//  * k_ms_push + k_ms_heavy: top-down push over the frontier queue for the
+//    bits in top-down mode (64-bit atomic ORs); long lists go to a
+//    load-balanced grid tier.
+//  * k_ms_pull: bottom-up pull over the vertices of graphs with bits in
+//    bottom-up mode (each vertex scans its neighbours until every wanted bit
+//    is found; long lists are scanned by a whole wave).  A young query's bit
+//    is never in a bottom-up want set, so the pull's early exit keeps
+//    working while queries of every age share it.
+//  * k_ms_count: per (graph, slot) count and degree sum of the vertices
+//    discovered, by a 64x64 bit transpose across each wave.
+//  * k_ms_direction: per slot, Beamer's rule (Beamer, Asanovic, Patterson,
+//    SC 2012) on the slot's own counts -> next level's bottom-up masks, and
+//    the per-graph "found anything" mask the host uses to retire queries.
 #include <algorithm>
 #include <cstring>
 #include <deque>
@@ -32,12 +44,12 @@ namespace b40c {
 
 struct alignas(256) MsCounters {
     uint32_t qlen[4];
-    unsigned long long level_edges[4];
-    uint32_t dir[4];
+    uint32_t any_bu[2];  // some slot is in bottom-up mode (by level parity)
     uint32_t overflow;
     uint32_t corrupt;
     uint32_t ph_base;
-    uint32_t pad;
+    uint32_t pad[7];
+    unsigned long long heavy_alloc[2];  // (entries << 32 | chunks) deferred to the grid tier, by parity
 };
 
 struct MsArgs {
@@ -45,17 +57,27 @@ struct MsArgs {
     const int32_t* cols;
     const int32_t* gid;
     const int32_t* vbase;
+    const long long* ebase;            // [G+1]
     unsigned long long* visited;
     unsigned long long* front[3];
-    const unsigned long long* active;  // [G] slots in use per graph
-    unsigned long long* newbits;       // [2][G] per-level discoveries by parity
-    const int32_t* bitq;               // [G*64] query in each slot (-1 free)
-    const long long* lbase;            // [Q] label slot base
-    const int32_t* qstart;             // [Q] level the query joined
+    unsigned long long* active;        // [G] slots in use per graph
+    unsigned long long* bu;            // [G] slots in bottom-up mode this level
+    unsigned long long* newbits;       // [G] slots that discovered vertices in the last level
+    uint32_t* cnt_n;                   // [G*64] vertices discovered in the last level, per slot
+    unsigned long long* cnt_m;         // [G*64] their degree sum
+    uint32_t* nf_prev;                 // [G*64] the level before's count
+    unsigned long long* m_seen;        // [G*64] degree sum of everything discovered so far
+    int32_t* bitq;                     // [G*64] query in each slot (-1 free)
+    long long* lbase;                  // [Q] label slot base
+    int32_t* qstart;                   // [Q] level the query joined
     int32_t* depth_out;
     int32_t* pred_out;
     int32_t* queue[2];
+    int2* heavy;                       // grid-tier entries (vertex, first chunk)
+    int32_t* chunk_map;                // chunk -> entry
     uint32_t qcap;
+    uint32_t heavy_cap;
+    uint32_t chunk_cap;
     MsCounters* ctr;
     int32_t n;
     int32_t G;
@@ -63,13 +85,52 @@ struct MsArgs {
     int32_t beta;
     int32_t cta_th;
     int32_t wave_th;
+    int32_t grid_th;
+    int32_t bu_wave_th;
     int32_t dev_base;
-    long long active_edges;
-    long long active_vertices;
 };
 
 __device__ __forceinline__ unsigned long long atomic_or_u64(unsigned long long* p, unsigned long long v) {
     return __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void atomic_and_u64(unsigned long long* p, unsigned long long v) {
+    (void)__hip_atomic_fetch_and(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ unsigned long long wave_or64(unsigned long long x) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) x |= __shfl_xor(x, d, kWave);
+    return x;
+}
+
+// OR of x over the lanes below this one.
+__device__ __forceinline__ unsigned long long wave_excl_or64(unsigned long long x) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int d = 1; d < kWave; d <<= 1) {
+        const unsigned long long y = __shfl_up(x, d, kWave);
+        if (lane >= d) x |= y;
+    }
+    const unsigned long long e = __shfl_up(x, 1, kWave);
+    return lane == 0 ? 0ull : e;
+}
+
+// 64x64 bit-matrix transpose across a wave: on return, bit i of lane b's
+// word is bit b of lane i's input.  Six block-swap stages of width 32..1.
+__device__ __forceinline__ unsigned long long wave_transpose64(unsigned long long x) {
+    const int lane = lane_id();
+    const unsigned long long lo[6] = {0x00000000FFFFFFFFull, 0x0000FFFF0000FFFFull, 0x00FF00FF00FF00FFull,
+                                      0x0F0F0F0F0F0F0F0Full, 0x3333333333333333ull, 0x5555555555555555ull};
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const int s = 32 >> k;
+        const unsigned long long p = __shfl_xor(x, s, kWave);
+        if (lane & s)
+            x = (x & ~lo[k]) | ((p & ~lo[k]) >> s);
+        else
+            x = (x & lo[k]) | ((p & lo[k]) << s);
+    }
+    return x;
 }
 
 struct MsShared {
@@ -94,148 +155,372 @@ __device__ __forceinline__ void ms_record(const MsArgs& a, int ph, int32_t v, in
     }
 }
 
-// Wave-aggregated OR of per-lane discoveries into newbits[slot][g]: lanes
-// sharing lane 0's graph are reduced with shuffles, the rest go direct.
-__device__ __forceinline__ void ms_note_new(const MsArgs& a, int slot, int32_t g, unsigned long long bits) {
-    const int32_t g0 = __shfl(g, 0, kWave);
-    const bool same = (g == g0);
-    unsigned long long x = same ? bits : 0ull;
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) x |= __shfl_xor(x, d, kWave);
-    if (lane_id() == 0 && x && g0 >= 0) atomic_or_u64(&a.newbits[(size_t)slot * a.G + g0], x);
-    if (!same && bits && g >= 0) atomic_or_u64(&a.newbits[(size_t)slot * a.G + g], bits);
-}
-
-__device__ __forceinline__ uint32_t ms_direction(const MsArgs& a, int ph, uint32_t n_f) {
-    const long long m_f = (long long)a.ctr->level_edges[ph & 3];
-    const uint32_t prev = a.ctr->dir[(ph + 3) & 3];
-    if (prev == 0) return (m_f * (long long)a.alpha > a.active_edges) ? 1u : 0u;
-    return ((long long)n_f * (long long)a.beta < a.active_vertices) ? 0u : 1u;
-}
-
-__device__ __forceinline__ void ms_phase_begin(const MsArgs& a, int ph) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        st_relaxed(&a.ctr->qlen[(ph + 2) & 3], 0u);
-        a.ctr->level_edges[(ph + 2) & 3] = 0ull;
-    }
-}
-
-__global__ __launch_bounds__(BFS_THREADS) void k_ms_level(MsArgs a, int ph) {
-    __shared__ MsShared s;
-    if (a.dev_base) ph += (int)ld_relaxed(&a.ctr->ph_base);
-    uint32_t n_f = ld_relaxed(&a.ctr->qlen[ph & 3]);
-    if (n_f > a.qcap) n_f = 0;
-    const int nslot = (ph + 1) & 1;
-    // Clear next level's new-bit slot (read by the host after this level) and
-    // the mask array two levels ahead.
-    for (int32_t i = blockIdx.x * BFS_THREADS + threadIdx.x; i < a.G; i += gridDim.x * BFS_THREADS)
-        a.newbits[(size_t)nslot * a.G + i] = 0ull;
-    unsigned long long* clear = a.front[(ph + 2) % 3];
-    for (int32_t v = blockIdx.x * BFS_THREADS + threadIdx.x; v < a.n; v += gridDim.x * BFS_THREADS) clear[v] = 0ull;
-    if (n_f == 0) {
-        ms_phase_begin(a, ph);
-        return;
-    }
-    const uint32_t dir = ms_direction(a, ph, n_f);
-    if (blockIdx.x == 0 && threadIdx.x == 0) a.ctr->dir[ph & 3] = dir;
-    if (threadIdx.x == 0) {
-        s.q.count = 0;
-    }
-    gather_init(s.g);
-    __syncthreads();
-    ms_phase_begin(a, ph);
+__device__ __forceinline__ GlobalQueue ms_out_queue(const MsArgs& a, int ph) {
     GlobalQueue out;
     out.keys = a.queue[(ph + 1) & 1];
     out.vals = nullptr;
     out.len = &a.ctr->qlen[(ph + 1) & 3];
     out.cap = a.qcap;
     out.overflow = &a.ctr->overflow;
+    return out;
+}
+
+// Top-down edge (src -> v) for the bits of src in top-down mode: exact
+// claim by atomic OR on visited[v], labels, and the first OR into next[v]
+// enqueues v.  Wave-collective (the LDS queue append).
+__device__ __forceinline__ void ms_push_edge(const MsArgs& a, int ph, MsShared& s, const GlobalQueue& out,
+                                             bool valid, int32_t v, int32_t src) {
+    const unsigned long long* cur = a.front[ph % 3];
+    unsigned long long nb = 0;
+    if (valid) {
+        const unsigned long long w = cur[src] & ~a.bu[a.gid[src]] & ~a.visited[v];
+        if (w) {
+            const unsigned long long old = atomic_or_u64(&a.visited[v], w);
+            nb = w & ~old;
+        }
+    }
+    bool enq = false;
+    if (nb) {
+        ms_record(a, ph, v, src, nb);
+        enq = atomic_or_u64(&a.front[(ph + 1) % 3][v], nb) == 0ull;
+    }
+    lq_append(s.q, out, enq, v, 0);
+}
+
+// Grid tier: a list of at least grid_th edges is cut into HEAVY_CHUNK-edge
+// chunks taken by all workgroups of k_ms_heavy.  One 64-bit atomic per wave
+// allocates (entries << 32 | chunks); each chunk points at its entry.
+__device__ __forceinline__ void ms_defer_heavy(const MsArgs& a, int ph, int32_t v, int32_t& deg) {
+    const bool heavy = deg >= a.grid_th;
+    const uint64_t m = __ballot(heavy);
+    if (m == 0) return;
+    const int lane = lane_id();
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    const uint32_t nch = heavy ? (uint32_t)((deg + HEAVY_CHUNK - 1) / HEAVY_CHUNK) : 0u;
+    const uint32_t ch_incl = (uint32_t)wave_inclusive_scan((int)nch);
+    const uint32_t ch_total = (uint32_t)__shfl((int)ch_incl, kWave - 1, kWave);
+    unsigned long long base = 0;
+    if (lane == leader)
+        base = __hip_atomic_fetch_add(&a.ctr->heavy_alloc[ph & 1],
+                                      ((unsigned long long)__popcll(m) << 32) | (unsigned long long)ch_total,
+                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t base_lo = (uint32_t)wave_bcast((int)(uint32_t)base, leader);
+    const uint32_t base_hi = (uint32_t)wave_bcast((int)(uint32_t)(base >> 32), leader);
+    if (heavy) {
+        const uint32_t entry = base_hi + (uint32_t)lane_prefix(m);
+        const uint32_t c0 = base_lo + ch_incl - nch;
+        if (entry < a.heavy_cap && c0 + nch <= a.chunk_cap) {
+            a.heavy[entry] = make_int2(v, (int)c0);
+            for (uint32_t c = 0; c < nch; ++c) a.chunk_map[c0 + c] = (int32_t)entry;
+        } else {
+            st_relaxed(&a.ctr->overflow, 1u);
+        }
+        deg = 0;
+    }
+}
+
+// Top-down push of level ph (lists below the grid tier's threshold).
+// Vertices that gain bits are appended once (the first OR into next[v]
+// enqueues) to the next frontier queue.
+__global__ __launch_bounds__(BFS_THREADS) void k_ms_push(MsArgs a, int ph) {
+    __shared__ MsShared s;
+    if (a.dev_base) ph += (int)ld_relaxed(&a.ctr->ph_base);
+    uint32_t n_f = ld_relaxed(&a.ctr->qlen[ph & 3]);
+    if (n_f > a.qcap) n_f = 0;
+    // Clear the mask array two levels ahead, its queue length and the next
+    // level's grid-tier allocation.
+    unsigned long long* clear = a.front[(ph + 2) % 3];
+    for (int32_t v = blockIdx.x * BFS_THREADS + threadIdx.x; v < a.n; v += gridDim.x * BFS_THREADS) clear[v] = 0ull;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st_relaxed(&a.ctr->qlen[(ph + 2) & 3], 0u);
+        __hip_atomic_store(&a.ctr->heavy_alloc[(ph + 1) & 1], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (n_f == 0) return;  // no vertex carries a bit: nothing can be found
+    if (threadIdx.x == 0) s.q.count = 0;
+    gather_init(s.g);
+    __syncthreads();
+    const GlobalQueue out = ms_out_queue(a, ph);
+    const unsigned long long* cur = a.front[ph % 3];
+
+    // Top-down push over the frontier queue, for the bits in top-down mode.
+    const int32_t* qin = a.queue[ph & 1];
+    for (uint32_t base = blockIdx.x * BFS_THREADS; base < n_f; base += gridDim.x * BFS_THREADS) {
+        const uint32_t i = base + threadIdx.x;
+        int32_t u = -1, start = 0, deg = 0;
+        if (i < n_f) {
+            const int32_t x = qin[i];
+            if ((uint32_t)x < (uint32_t)a.n) {
+                if (cur[x] & ~a.bu[a.gid[x]]) {
+                    u = x;
+                    start = a.offs[x];
+                    deg = a.offs[x + 1] - start;
+                }
+            } else {
+                st_relaxed(&a.ctr->corrupt, 1u);
+            }
+        }
+        ms_defer_heavy(a, ph, u, deg);
+        auto sink = [&](bool valid, int32_t v, int32_t src) { ms_push_edge(a, ph, s, out, valid, v, src); };
+        tiered_gather(a.cols, u, start, deg, a.cta_th, a.wave_th, s.g, sink);
+        lq_flush(s.q, out, LQ_CAP / 2);
+    }
+    lq_flush(s.q, out, 1);
+}
+
+// Bottom-up pull of level ph for the bits in bottom-up mode: each vertex of
+// such a graph scans its neighbours until every wanted bit is found.  Bits
+// are disjoint from the push's, but both OR into visited[v] and next[v], so
+// both use atomics there.
+struct MsPullShared {
+    LdsQueue<false> q;
+};
+
+__global__ __launch_bounds__(BFS_THREADS) void k_ms_pull(MsArgs a, int ph) {
+    __shared__ MsPullShared s;
+    if (a.dev_base) ph += (int)ld_relaxed(&a.ctr->ph_base);
+    uint32_t n_f = ld_relaxed(&a.ctr->qlen[ph & 3]);
+    if (n_f > a.qcap) n_f = 0;
+    if (n_f == 0 || !ld_relaxed(&a.ctr->any_bu[ph & 1])) return;
+    if (threadIdx.x == 0) s.q.count = 0;
+    __syncthreads();
+    const GlobalQueue out = ms_out_queue(a, ph);
     const unsigned long long* cur = a.front[ph % 3];
     unsigned long long* next = a.front[(ph + 1) % 3];
-    const int slot = ph & 1;  // newbits slot written by this level
-    unsigned long long eacc = 0;
-
-    if (dir == 0) {
-        // Top-down push over the frontier queue.
-        const int32_t* qin = a.queue[ph & 1];
-        for (uint32_t base = blockIdx.x * BFS_THREADS; base < n_f; base += gridDim.x * BFS_THREADS) {
-            const uint32_t i = base + threadIdx.x;
-            int32_t u = -1, start = 0, deg = 0;
-            if (i < n_f) {
-                u = qin[i];
-                if ((uint32_t)u < (uint32_t)a.n) {
-                    start = a.offs[u];
-                    deg = a.offs[u + 1] - start;
-                } else {
-                    st_relaxed(&a.ctr->corrupt, 1u);
-                    u = -1;
-                }
-            }
-            auto sink = [&](bool valid, int32_t v, int32_t src) {
-                unsigned long long nb = 0;
-                int32_t g = -1;
-                if (valid) {
-                    const unsigned long long w = cur[src] & ~a.visited[v];
-                    if (w) {
-                        const unsigned long long old = atomic_or_u64(&a.visited[v], w);
-                        nb = w & ~old;
-                    }
-                }
-                bool enq = false;
-                if (nb) {
-                    g = a.gid[v];
-                    ms_record(a, ph, v, src, nb);
-                    enq = atomic_or_u64(&next[v], nb) == 0ull;
-                    if (enq) eacc += (unsigned long long)(a.offs[v + 1] - a.offs[v]);
-                }
-                ms_note_new(a, slot, g, nb);
-                lq_append(s.q, out, enq, v, 0);
-            };
-            tiered_gather(a.cols, u, start, deg, a.cta_th, a.wave_th, s.g, sink);
-            lq_flush(s.q, out, LQ_CAP / 2);
-        }
-    } else {
-        // Bottom-up pull over every vertex with bits still wanted.
-        for (int32_t v = blockIdx.x * BFS_THREADS + threadIdx.x; v - (int32_t)threadIdx.x < a.n;
-             v += gridDim.x * BFS_THREADS) {
-            unsigned long long got = 0;
-            int32_t g = -1;
+    {
+        const int lane = lane_id();
+        const int wid = threadIdx.x / kWave;
+        const int64_t nwaves = (int64_t)gridDim.x * BFS_WAVES;
+        for (int64_t vb = ((int64_t)blockIdx.x * BFS_WAVES + wid) * kWave; vb < a.n; vb += nwaves * kWave) {
+            const int32_t v = (int32_t)(vb + lane);
+            unsigned long long want = 0, got = 0;
+            int32_t start = 0, deg = 0;
             if (v < a.n) {
-                g = a.gid[v];
-                const unsigned long long vis = a.visited[v];
-                const unsigned long long want = a.active[g] & ~vis;
-                if (want) {
-                    const int32_t e1 = a.offs[v + 1];
-                    for (int32_t e = a.offs[v]; e < e1; ++e) {
-                        const int32_t u = a.cols[e];
-                        const unsigned long long f = cur[u] & want & ~got;
-                        if (f) {
-                            got |= f;
-                            ms_record(a, ph, v, u, f);
-                            if (got == want) break;
-                        }
-                    }
-                    if (got) {
-                        a.visited[v] = vis | got;
-                        next[v] = got;
-                        eacc += (unsigned long long)(e1 - a.offs[v]);
+                const unsigned long long m = a.bu[a.gid[v]];
+                if (m) {
+                    want = m & ~a.visited[v];
+                    if (want) {
+                        start = a.offs[v];
+                        deg = a.offs[v + 1] - start;
                     }
                 }
             }
-            ms_note_new(a, slot, g, got);
-            lq_append(s.q, out, got != 0ull, v, 0);
+            // Long lists: the whole wave scans one list 64 neighbours at a
+            // time; each newly found bit is recorded by the lowest lane that
+            // holds it (the first such neighbour in list order).
+            uint64_t lm = __ballot(want != 0ull && deg > a.bu_wave_th);
+            while (lm) {
+                const int leader = __ffsll((unsigned long long)lm) - 1;
+                lm &= lm - 1;
+                const int32_t ls = wave_bcast(start, leader), ld = wave_bcast(deg, leader);
+                const unsigned long long lwant = __shfl(want, leader, kWave);
+                const int32_t lv = (int32_t)vb + leader;
+                unsigned long long lgot = 0;
+                for (int32_t e0 = 0; e0 < ld && lgot != lwant; e0 += kWave) {
+                    const int32_t e = e0 + lane;
+                    int32_t u = -1;
+                    unsigned long long f = 0;
+                    if (e < ld) {
+                        u = a.cols[ls + e];
+                        f = cur[u] & lwant & ~lgot;
+                    }
+                    const unsigned long long mine = f & ~wave_excl_or64(f);
+                    if (mine) ms_record(a, ph, lv, u, mine);
+                    lgot |= wave_or64(f);
+                }
+                if (lane == leader) got = lgot;
+            }
+            // Short lists: each lane scans its own, four neighbours per step.
+            if (want != 0ull && deg <= a.bu_wave_th) {
+                int32_t e = start;
+                const int32_t end = start + deg;
+                for (; e + 4 <= end && got != want; e += 4) {
+                    const int32_t u0 = a.cols[e], u1 = a.cols[e + 1], u2 = a.cols[e + 2], u3 = a.cols[e + 3];
+                    const unsigned long long c0 = cur[u0], c1 = cur[u1], c2 = cur[u2], c3 = cur[u3];
+                    unsigned long long f = c0 & want & ~got;
+                    if (f) {
+                        got |= f;
+                        ms_record(a, ph, v, u0, f);
+                    }
+                    f = c1 & want & ~got;
+                    if (f) {
+                        got |= f;
+                        ms_record(a, ph, v, u1, f);
+                    }
+                    f = c2 & want & ~got;
+                    if (f) {
+                        got |= f;
+                        ms_record(a, ph, v, u2, f);
+                    }
+                    f = c3 & want & ~got;
+                    if (f) {
+                        got |= f;
+                        ms_record(a, ph, v, u3, f);
+                    }
+                }
+                for (; e < end && got != want; ++e) {
+                    const int32_t u = a.cols[e];
+                    const unsigned long long f = cur[u] & want & ~got;
+                    if (f) {
+                        got |= f;
+                        ms_record(a, ph, v, u, f);
+                    }
+                }
+            }
+            bool enq = false;
+            if (got) {
+                atomic_or_u64(&a.visited[v], got);
+                enq = atomic_or_u64(&next[v], got) == 0ull;
+            }
+            lq_append(s.q, out, enq, v, 0);
         }
     }
     lq_flush(s.q, out, 1);
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) eacc += __shfl_xor(eacc, d, kWave);
+}
+
+// Grid tier of level ph's push: workgroups take chunks grid-stride, one edge
+// per thread.  Each chunk is checked against the entry it names, so a stale
+// map word never indexes outside a list.
+__global__ __launch_bounds__(BFS_THREADS) void k_ms_heavy(MsArgs a, int ph) {
+    __shared__ MsShared s;
+    if (a.dev_base) ph += (int)ld_relaxed(&a.ctr->ph_base);
+    const unsigned long long alloc =
+        __hip_atomic_load(&a.ctr->heavy_alloc[ph & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t H = min((uint32_t)(alloc >> 32), a.heavy_cap);
+    const uint32_t C = min((uint32_t)alloc, a.chunk_cap);
+    if (H == 0 || C == 0) return;
+    if (threadIdx.x == 0) s.q.count = 0;
     __syncthreads();
-    if (lane_id() == 0) s.red[threadIdx.x / kWave] = eacc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long t = 0;
-        for (int w = 0; w < BFS_WAVES; ++w) t += s.red[w];
-        if (t) atomic_add_u64(&a.ctr->level_edges[(ph + 1) & 3], t);
+    const GlobalQueue out = ms_out_queue(a, ph);
+    for (uint32_t c = blockIdx.x; c < C; c += gridDim.x) {
+        const int32_t entry = a.chunk_map[c];
+        int32_t u = -1, e = 0;
+        bool valid = false;
+        if ((uint32_t)entry < H) {
+            const int2 eu = a.heavy[entry];
+            if ((uint32_t)eu.x < (uint32_t)a.n && (uint32_t)eu.y <= c) {
+                const int32_t s0 = a.offs[eu.x], deg = a.offs[eu.x + 1] - s0;
+                const long long off = (long long)(c - (uint32_t)eu.y) * HEAVY_CHUNK + threadIdx.x;
+                if (off < deg) {
+                    u = eu.x;
+                    e = s0 + (int32_t)off;
+                    valid = true;
+                }
+            } else {
+                st_relaxed(&a.ctr->corrupt, 1u);
+            }
+        }
+        const int32_t v = valid ? a.cols[e] : -1;
+        ms_push_edge(a, ph, s, out, valid, v, u);
+        lq_flush(s.q, out, LQ_CAP / 2);
+    }
+    lq_flush(s.q, out, 1);
+}
+
+// Per (graph, slot) count and degree sum of the vertices discovered by level
+// ph (the next frontier queue; next[v] holds exactly v's new bits).  Each
+// wave takes one contiguous stretch of the queue (entries appended together
+// mostly share a graph) and, per chunk of 64 entries and per graph present in
+// it, transposes the lanes' bit masks so that lane b holds the set of lanes
+// carrying bit b.  Running totals are kept while the graph stays the same.
+__global__ __launch_bounds__(BFS_THREADS) void k_ms_count(MsArgs a, int ph) {
+    if (a.dev_base) ph += (int)ld_relaxed(&a.ctr->ph_base);
+    uint32_t n_q = ld_relaxed(&a.ctr->qlen[(ph + 1) & 3]);
+    if (n_q > a.qcap) n_q = 0;
+    const int32_t* q = a.queue[(ph + 1) & 1];
+    const unsigned long long* nxt = a.front[(ph + 1) % 3];
+    const int lane = lane_id();
+    const int64_t wave = (int64_t)blockIdx.x * BFS_WAVES + threadIdx.x / kWave;
+    const int64_t nwaves = (int64_t)gridDim.x * BFS_WAVES;
+    const int64_t per = (((int64_t)n_q + nwaves - 1) / nwaves + kWave - 1) / kWave * kWave;
+    const int64_t lo = wave * per, hi = min((int64_t)n_q, lo + per);
+    int32_t accg = -1;
+    uint32_t accn = 0;
+    unsigned long long accm = 0;
+    for (int64_t base = lo; base < hi; base += kWave) {
+        const int64_t i = base + lane;
+        int32_t g = -1, deg = 0;
+        unsigned long long bits = 0;
+        if (i < hi) {
+            const int32_t v = q[i];
+            if ((uint32_t)v < (uint32_t)a.n) {
+                g = a.gid[v];
+                bits = nxt[v];
+                deg = a.offs[v + 1] - a.offs[v];
+            } else {
+                st_relaxed(&a.ctr->corrupt, 1u);
+            }
+        }
+        bool todo = g >= 0;
+        for (uint64_t rest = __ballot(todo); rest; rest = __ballot(todo)) {
+            const int32_t g1 = __shfl(g, __ffsll((unsigned long long)rest) - 1, kWave);
+            const bool mine = todo && g == g1;
+            todo = todo && !mine;
+            const unsigned long long y = wave_transpose64(mine ? bits : 0ull);
+            unsigned long long m = 0;
+#pragma unroll 8
+            for (int l = 0; l < kWave; ++l) {
+                const int d = __shfl(deg, l, kWave);
+                if ((y >> l) & 1ull) m += (unsigned long long)d;
+            }
+            if (g1 != accg) {
+                if (accg >= 0 && accn) {
+                    atomic_add_u32(&a.cnt_n[(size_t)accg * 64 + lane], accn);
+                    atomic_add_u64(&a.cnt_m[(size_t)accg * 64 + lane], accm);
+                }
+                accg = g1;
+                accn = 0;
+                accm = 0;
+            }
+            accn += (uint32_t)__popcll(y);
+            accm += m;
+        }
+    }
+    if (accg >= 0 && accn) {
+        atomic_add_u32(&a.cnt_n[(size_t)accg * 64 + lane], accn);
+        atomic_add_u64(&a.cnt_m[(size_t)accg * 64 + lane], accm);
+    }
+}
+
+// One wave per graph, lane b = slot b.  The slot's level counts decide its
+// direction for level ph + 1: top-down -> bottom-up when the new frontier's
+// degree sum m_f exceeds m_u / alpha (m_u = edges of the graph not yet
+// reached by this query), bottom-up -> top-down when the frontier has shrunk
+// below V_g / beta.  Also publishes the "found anything" mask per graph and
+// consumes (zeroes) the counts.
+__global__ __launch_bounds__(BFS_THREADS) void k_ms_direction(MsArgs a, int ph) {
+    if (a.dev_base) ph += (int)ld_relaxed(&a.ctr->ph_base);
+    const int32_t g = (int32_t)(blockIdx.x * BFS_WAVES + threadIdx.x / kWave);
+    if (blockIdx.x == 0 && threadIdx.x == 0) st_relaxed(&a.ctr->any_bu[ph & 1], 0u);
+    if (g >= a.G) return;
+    const int b = lane_id();
+    const size_t k = (size_t)g * 64 + b;
+    const uint32_t n = a.cnt_n[k];
+    const unsigned long long m = a.cnt_m[k];
+    a.cnt_n[k] = 0u;
+    a.cnt_m[k] = 0ull;
+    const bool occupied = (a.active[g] >> b) & 1ull;
+    const bool was_bu = (a.bu[g] >> b) & 1ull;
+    const long long Vg = a.vbase[g + 1] - a.vbase[g];
+    const long long Eg = a.ebase[g + 1] - a.ebase[g];
+    bool to_bu = false;
+    if (occupied && n) {
+        const unsigned long long seen = a.m_seen[k] + m;
+        a.m_seen[k] = seen;
+        if (!was_bu) {
+            long long m_u = Eg - (long long)seen;
+            if (m_u < 0) m_u = 0;
+            to_bu = (long long)m * (long long)a.alpha > m_u;
+        } else {
+            to_bu = !((long long)n * (long long)a.beta < Vg && n < a.nf_prev[k]);
+        }
+    }
+    a.nf_prev[k] = n;
+    const unsigned long long bum = __ballot(to_bu);
+    const unsigned long long found = __ballot(occupied && n != 0u);
+    if (b == 0) {
+        a.bu[g] = bum;
+        a.newbits[g] = found;
+        if (bum) st_relaxed(&a.ctr->any_bu[(ph + 1) & 1], 1u);
     }
 }
 
@@ -253,26 +538,30 @@ struct MsSeed {
 };
 
 // Registers newly admitted queries (slot table, active mask, label base,
-// start level) and seeds them into level ph: source bit in visited and the
-// current mask array, source labels, and the source appended to the current
-// frontier queue if it was not already on it.  The slot table and masks are
-// only written here and in k_ms_release, never copied from the host.
-__global__ void k_ms_seed(MsArgs a, int ph, const MsSeed* seeds, int count, int32_t* bitq,
-                          unsigned long long* active, long long* lbase, int32_t* qstart) {
+// start level, top-down mode, fresh direction statistics) and seeds them
+// into level ph: source bit in visited and the current mask array, source
+// labels, and the source appended to the current frontier queue if it was
+// not already on it.  The slot table and masks are only written here, in
+// k_ms_release and k_ms_direction, never copied from the host.
+__global__ void k_ms_seed(MsArgs a, int ph, const MsSeed* seeds, int count) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= count) return;
     const MsSeed sd = seeds[i];
     const int32_t q = sd.q, v = sd.src;
-    if ((uint32_t)v >= (uint32_t)a.n) {
+    if ((uint32_t)v >= (uint32_t)a.n || (uint32_t)sd.bit >= 64u) {
         st_relaxed(&a.ctr->corrupt, 1u);
         return;
     }
     const unsigned long long bit = 1ull << sd.bit;
     const int32_t g = a.gid[v];
-    bitq[g * 64 + sd.bit] = q;
-    lbase[q] = sd.lbase;
-    qstart[q] = sd.qstart;
-    atomic_or_u64(&active[g], bit);
+    const size_t k = (size_t)g * 64 + sd.bit;
+    a.bitq[k] = q;
+    a.lbase[q] = sd.lbase;
+    a.qstart[q] = sd.qstart;
+    a.nf_prev[k] = 1u;
+    a.m_seen[k] = 0ull;
+    atomic_and_u64(&a.bu[g], ~bit);
+    atomic_or_u64(&a.active[g], bit);
     atomic_or_u64(&a.visited[v], bit);
     const long long idx = sd.lbase + (v - a.vbase[g]);
     a.depth_out[idx] = 0;
@@ -284,21 +573,20 @@ __global__ void k_ms_seed(MsArgs a, int ph, const MsSeed* seeds, int count, int3
             a.queue[ph & 1][pos] = v;
         else
             st_relaxed(&a.ctr->overflow, 1u);
-        atomic_add_u64(&a.ctr->level_edges[ph & 3], (unsigned long long)(a.offs[v + 1] - a.offs[v]));
     }
 }
 
-// Releases graph g's slot `bit`: slot table entry, active bit, and the bit in
-// visited over the graph's vertex range [v0, v1).
-__global__ void k_ms_release(unsigned long long* visited, int32_t v0, int32_t v1, int32_t g, int32_t bit,
-                             int32_t* bitq, unsigned long long* active) {
+// Releases graph g's slot `bit`: slot table entry, active and bottom-up
+// bits, and the bit in visited over the graph's vertex range [v0, v1).
+__global__ void k_ms_release(MsArgs a, int32_t v0, int32_t v1, int32_t g, int32_t bit) {
     const unsigned long long keep = ~(1ull << bit);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        bitq[g * 64 + bit] = -1;
-        __hip_atomic_fetch_and(&active[g], keep, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        a.bitq[(size_t)g * 64 + bit] = -1;
+        atomic_and_u64(&a.active[g], keep);
+        atomic_and_u64(&a.bu[g], keep);
     }
     for (int32_t v = v0 + blockIdx.x * blockDim.x + threadIdx.x; v < v1; v += gridDim.x * blockDim.x)
-        visited[v] &= keep;
+        a.visited[v] &= keep;
 }
 
 __global__ void k_fill_i32(int32_t* p, int32_t value, long long n) {
@@ -330,7 +618,7 @@ struct b40c_multigraph {
     int32_t ngraphs = 0;
     std::vector<int32_t> vbase;  // ngraphs + 1
     std::vector<int64_t> ebase;  // ngraphs + 1
-    b40c::DeviceBuffer d_vbase, d_gid;
+    b40c::DeviceBuffer d_vbase, d_ebase, d_gid;
 };
 
 namespace b40c {
@@ -389,11 +677,12 @@ struct b40c_msbfs {
     hipStream_t stream = nullptr;
     int32_t n = 0, G = 0;
     int64_t arena_elems = 0;  // per label array (depth and pred each)
-    b40c::DeviceBuffer visited, front[3], active, newbits, bitq, lbase, qstart, depth, pred, queue[2], counters;
-    b40c::DeviceBuffer seed_buf;
+    b40c::DeviceBuffer visited, front[3], active, bu, newbits, cnt_n, cnt_m, nf_prev, m_seen, bitq, lbase, qstart, depth,
+        pred, queue[2], counters;
+    b40c::DeviceBuffer seed_buf, heavy, chunk_map;
     b40c::MsSeed* h_seeds = nullptr;  // pinned staging
     int h_seed_cap = 0;
-    uint32_t qcap = 0;
+    uint32_t qcap = 0, heavy_cap = 0, chunk_cap = 0;
     b40c::ArenaAlloc arena;
     std::vector<b40c::MsQuery> queries;
     std::deque<int32_t> waiting;
@@ -416,10 +705,16 @@ MsArgs ms_args(b40c_msbfs* m) {
     a.cols = m->mg->packed->d_cols();
     a.gid = m->mg->d_gid.as<int32_t>();
     a.vbase = m->mg->d_vbase.as<int32_t>();
+    a.ebase = m->mg->d_ebase.as<long long>();
     a.visited = m->visited.as<unsigned long long>();
     for (int i = 0; i < 3; ++i) a.front[i] = m->front[i].as<unsigned long long>();
     a.active = m->active.as<unsigned long long>();
+    a.bu = m->bu.as<unsigned long long>();
     a.newbits = m->newbits.as<unsigned long long>();
+    a.cnt_n = m->cnt_n.as<uint32_t>();
+    a.cnt_m = m->cnt_m.as<unsigned long long>();
+    a.nf_prev = m->nf_prev.as<uint32_t>();
+    a.m_seen = m->m_seen.as<unsigned long long>();
     a.bitq = m->bitq.as<int32_t>();
     a.lbase = m->lbase.as<long long>();
     a.qstart = m->qstart.as<int32_t>();
@@ -428,6 +723,10 @@ MsArgs ms_args(b40c_msbfs* m) {
     a.queue[0] = m->queue[0].as<int32_t>();
     a.queue[1] = m->queue[1].as<int32_t>();
     a.qcap = m->qcap;
+    a.heavy = m->heavy.as<int2>();
+    a.chunk_map = m->chunk_map.as<int32_t>();
+    a.heavy_cap = m->heavy_cap;
+    a.chunk_cap = m->chunk_cap;
     a.ctr = m->counters.as<MsCounters>();
     a.n = m->n;
     a.G = m->G;
@@ -435,15 +734,9 @@ MsArgs ms_args(b40c_msbfs* m) {
     a.beta = m->beta;
     a.cta_th = 256;
     a.wave_th = 64;
+    a.grid_th = 128;
+    a.bu_wave_th = 32;
     a.dev_base = 0;
-    int64_t ae = 0, av = 0;
-    for (int g = 0; g < m->G; ++g)
-        if (m->h_active[g]) {
-            ae += m->mg->ebase[g + 1] - m->mg->ebase[g];
-            av += m->mg->vbase[g + 1] - m->mg->vbase[g];
-        }
-    a.active_edges = std::max<int64_t>(ae, 1);
-    a.active_vertices = std::max<int64_t>(av, 1);
     return a;
 }
 
@@ -500,21 +793,20 @@ b40c_status ms_admit(b40c_msbfs* m) {
     B40C_HIP(hipMemcpyAsync(m->seed_buf.ptr, m->h_seeds, sizeof(MsSeed) * cnt, hipMemcpyHostToDevice, m->stream));
     const MsArgs a = ms_args(m);
     hipLaunchKernelGGL(k_ms_seed, dim3((cnt + 255) / 256), dim3(256), 0, m->stream, a, m->ph,
-                       m->seed_buf.as<MsSeed>(), cnt, m->bitq.as<int32_t>(), m->active.as<unsigned long long>(),
-                       m->lbase.as<long long>(), m->qstart.as<int32_t>());
+                       m->seed_buf.as<MsSeed>(), cnt);
     B40C_HIP(hipGetLastError());
     return B40C_OK;
 }
 
 // Retires queries whose slot found nothing new in the level just run.
 b40c_status ms_retire(b40c_msbfs* m) {
-    const int slot = m->ph & 1;  // the level that just ran wrote this slot
     std::vector<int32_t> keep;
+    const MsArgs a = ms_args(m);
     for (int32_t q : m->running) {
         MsQuery& Q = m->queries[q];
         const unsigned long long bit = 1ull << Q.bit;
         Q.levels = m->ph + 1 - Q.qstart;
-        if (m->h_newbits[(size_t)slot * m->G + Q.graph] & bit) {
+        if (m->h_newbits[Q.graph] & bit) {
             keep.push_back(q);
             continue;
         }
@@ -522,9 +814,8 @@ b40c_status ms_retire(b40c_msbfs* m) {
         m->completed.push_back(q);
         m->h_active[Q.graph] &= ~bit;
         const int32_t v0 = m->mg->vbase[Q.graph], v1 = m->mg->vbase[Q.graph + 1];
-        hipLaunchKernelGGL(k_ms_release, dim3(grid_for_n(v1 - v0)), dim3(256), 0, m->stream,
-                           m->visited.as<unsigned long long>(), v0, v1, Q.graph, Q.bit, m->bitq.as<int32_t>(),
-                           m->active.as<unsigned long long>());
+        hipLaunchKernelGGL(k_ms_release, dim3(grid_for_n(v1 - v0)), dim3(256), 0, m->stream, a, v0, v1, Q.graph,
+                           Q.bit);
         B40C_HIP(hipGetLastError());
     }
     m->running.swap(keep);
@@ -565,6 +856,7 @@ b40c_status b40c_multigraph_pack(b40c_graph* const* graphs, int32_t ngraphs, b40
     if (s == B40C_OK) s = P->cols.alloc((size_t)std::max<int64_t>(M, 1) * 4);
     if (s == B40C_OK) s = mg->d_gid.alloc((size_t)N * 4);
     if (s == B40C_OK) s = mg->d_vbase.alloc(((size_t)ngraphs + 1) * 4);
+    if (s == B40C_OK) s = mg->d_ebase.alloc(((size_t)ngraphs + 1) * 8);
     if (s != B40C_OK) {
         delete P;
         delete mg;
@@ -583,6 +875,7 @@ b40c_status b40c_multigraph_pack(b40c_graph* const* graphs, int32_t ngraphs, b40
     for (int g = 0; g < ngraphs; ++g) all_sym = all_sym && graphs[g]->symmetric.load() == 1;
     P->symmetric.store(all_sym ? 1 : -1);
     hipError_t e = hipMemcpy(mg->d_vbase.ptr, mg->vbase.data(), mg->vbase.size() * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(mg->d_ebase.ptr, mg->ebase.data(), mg->ebase.size() * 8, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipDeviceSynchronize();
     if (e != hipSuccess) {
         delete P;
@@ -634,7 +927,12 @@ b40c_status b40c_msbfs_create(b40c_multigraph* mg, int64_t label_budget_bytes, i
     if (s == B40C_OK) s = m->visited.alloc(N * 8);
     for (int i = 0; i < 3 && s == B40C_OK; ++i) s = m->front[i].alloc(N * 8);
     if (s == B40C_OK) s = m->active.alloc((size_t)m->G * 8);
-    if (s == B40C_OK) s = m->newbits.alloc((size_t)m->G * 16);
+    if (s == B40C_OK) s = m->bu.alloc((size_t)m->G * 8);
+    if (s == B40C_OK) s = m->newbits.alloc((size_t)m->G * 8);
+    if (s == B40C_OK) s = m->cnt_n.alloc((size_t)m->G * 64 * 4);
+    if (s == B40C_OK) s = m->cnt_m.alloc((size_t)m->G * 64 * 8);
+    if (s == B40C_OK) s = m->nf_prev.alloc((size_t)m->G * 64 * 4);
+    if (s == B40C_OK) s = m->m_seen.alloc((size_t)m->G * 64 * 8);
     if (s == B40C_OK) s = m->bitq.alloc((size_t)m->G * 64 * 4);
     if (s == B40C_OK) s = m->lbase.alloc((size_t)max_queries * 8);
     if (s == B40C_OK) s = m->qstart.alloc((size_t)max_queries * 4);
@@ -643,16 +941,25 @@ b40c_status b40c_msbfs_create(b40c_multigraph* mg, int64_t label_budget_bytes, i
     if (s == B40C_OK) s = m->queue[0].alloc(N * 4);
     if (s == B40C_OK) s = m->queue[1].alloc(N * 4);
     if (s == B40C_OK) s = m->counters.alloc(sizeof(MsCounters));
+    m->heavy_cap = (uint32_t)m->n;
+    m->chunk_cap = (uint32_t)std::min<int64_t>(mg->packed->edges / HEAVY_CHUNK + (int64_t)m->n + 64, 0x7FFFFFFFll);
+    if (s == B40C_OK) s = m->heavy.alloc((size_t)m->heavy_cap * sizeof(int2));
+    if (s == B40C_OK) s = m->chunk_map.alloc((size_t)m->chunk_cap * 4);
     if (s != B40C_OK) {
         b40c_msbfs_destroy(m);
         return s;
     }
     m->arena.init(m->arena_elems);
     m->h_active.assign(m->G, 0ull);
-    m->h_newbits.assign((size_t)m->G * 2, 0ull);
+    m->h_newbits.assign((size_t)m->G, 0ull);
     B40C_HIP(hipMemsetAsync(m->visited.ptr, 0, m->visited.bytes, m->stream));
     for (int i = 0; i < 3; ++i) B40C_HIP(hipMemsetAsync(m->front[i].ptr, 0, m->front[i].bytes, m->stream));
     B40C_HIP(hipMemsetAsync(m->newbits.ptr, 0, m->newbits.bytes, m->stream));
+    B40C_HIP(hipMemsetAsync(m->bu.ptr, 0, m->bu.bytes, m->stream));
+    B40C_HIP(hipMemsetAsync(m->cnt_n.ptr, 0, m->cnt_n.bytes, m->stream));
+    B40C_HIP(hipMemsetAsync(m->cnt_m.ptr, 0, m->cnt_m.bytes, m->stream));
+    B40C_HIP(hipMemsetAsync(m->nf_prev.ptr, 0, m->nf_prev.bytes, m->stream));
+    B40C_HIP(hipMemsetAsync(m->m_seen.ptr, 0, m->m_seen.bytes, m->stream));
     B40C_HIP(hipMemsetAsync(m->counters.ptr, 0, sizeof(MsCounters), m->stream));
     B40C_HIP(hipMemsetAsync(m->bitq.ptr, 0xFF, m->bitq.bytes, m->stream));  // all slots free (-1)
     B40C_HIP(hipMemsetAsync(m->active.ptr, 0, m->active.bytes, m->stream));
@@ -695,7 +1002,12 @@ b40c_status b40c_msbfs_step(b40c_msbfs* m, int32_t* running, int32_t* waiting) {
     B40C_TRY(ms_admit(m));
     if (!m->running.empty()) {
         const MsArgs a = ms_args(m);
-        hipLaunchKernelGGL(k_ms_level, dim3(m->grid), dim3(BFS_THREADS), 0, m->stream, a, m->ph);
+        hipLaunchKernelGGL(k_ms_push, dim3(m->grid), dim3(BFS_THREADS), 0, m->stream, a, m->ph);
+        hipLaunchKernelGGL(k_ms_heavy, dim3(m->grid), dim3(BFS_THREADS), 0, m->stream, a, m->ph);
+        hipLaunchKernelGGL(k_ms_pull, dim3(2 * m->grid), dim3(BFS_THREADS), 0, m->stream, a, m->ph);
+        hipLaunchKernelGGL(k_ms_count, dim3(m->grid), dim3(BFS_THREADS), 0, m->stream, a, m->ph);
+        hipLaunchKernelGGL(k_ms_direction, dim3((m->G + BFS_WAVES - 1) / BFS_WAVES), dim3(BFS_THREADS), 0, m->stream,
+                           a, m->ph);
         B40C_HIP(hipGetLastError());
         B40C_HIP(hipMemcpyAsync(m->h_ctr, m->counters.ptr, sizeof(MsCounters), hipMemcpyDeviceToHost, m->stream));
         B40C_HIP(hipMemcpyAsync(m->h_newbits.data(), m->newbits.ptr, m->h_newbits.size() * 8, hipMemcpyDeviceToHost,
@@ -758,7 +1070,12 @@ void b40c_msbfs_destroy(b40c_msbfs* m) {
     m->visited.release();
     for (int i = 0; i < 3; ++i) m->front[i].release();
     m->active.release();
+    m->bu.release();
     m->newbits.release();
+    m->cnt_n.release();
+    m->cnt_m.release();
+    m->nf_prev.release();
+    m->m_seen.release();
     m->bitq.release();
     m->lbase.release();
     m->qstart.release();
@@ -766,6 +1083,8 @@ void b40c_msbfs_destroy(b40c_msbfs* m) {
     m->pred.release();
     m->queue[0].release();
     m->queue[1].release();
+    m->heavy.release();
+    m->chunk_map.release();
     m->counters.release();
     m->seed_buf.release();
     if (m->h_seeds) (void)hipHostFree(m->h_seeds);

@@ -1,0 +1,177 @@
+// C++ parity driver in the shape of the reference's test/bfs/test_bfs.cu
+// RunTests loop (test_bfs.cu:584-783): build a graph, then per source
+// Reset -> timed EnactSearch -> GetStatistics -> ExtractResults -> validate
+// against the serial CPU BFS (the oracle, test infrastructure) and the
+// predecessor invariants (test_bfs.cu:387-438).  It drives the library only
+// through the C++ facade (include/b40c/bfs.hpp), i.e. the way a
+// reference-side caller would after relinking.
+//
+// usage: test_bfs rmat <scale> [edge_factor] | grid3d <width>
+//                 [--strategy=0..5] [--mode=default|fused|iterative]
+//                 [--i=<sources>] [--src=<vertex>] [--seed=<n>]
+//                 [--queue-sizing=<f>]
+// Prints one line per source and a summary; exit status 0 iff every source
+// is CORRECT.
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "b40c/bfs.hpp"
+
+extern "C" {
+int32_t orc_bfs(int32_t n, const int32_t* offsets, const int32_t* cols, int32_t src, int32_t* depth, int32_t* pred);
+int64_t orc_check_pred(int32_t n, const int32_t* offsets, const int32_t* cols, const int32_t* ref_depth,
+                       const int32_t* pred);
+int64_t orc_edges_visited(int32_t n, const int32_t* offsets, const int32_t* labels, int64_t* nodes_visited);
+int32_t orc_pick_sources(int32_t n, const int32_t* offsets, uint64_t seed, int32_t count, int32_t* out);
+}
+
+using namespace b40c::graph::bfs;
+typedef CsrProblem<int, int, true> Problem;
+
+namespace {
+
+struct Options {
+    std::string kind;
+    int scale = 12, edge_factor = 16, width = 32;
+    int strategy = B40C_HYBRID;
+    std::string mode = "default";
+    int iterations = 4;
+    int src = -1;
+    unsigned long long seed = 1;
+    double queue_sizing = 1.3;
+};
+
+bool parse(int argc, char** argv, Options& o) {
+    int pos = 0;
+    for (int i = 1; i < argc; ++i) {
+        const char* a = argv[i];
+        if (!strncmp(a, "--strategy=", 11)) o.strategy = atoi(a + 11);
+        else if (!strncmp(a, "--mode=", 7)) o.mode = a + 7;
+        else if (!strncmp(a, "--i=", 4)) o.iterations = atoi(a + 4);
+        else if (!strncmp(a, "--src=", 6)) o.src = atoi(a + 6);
+        else if (!strncmp(a, "--seed=", 7)) o.seed = strtoull(a + 7, nullptr, 10);
+        else if (!strncmp(a, "--queue-sizing=", 15)) o.queue_sizing = atof(a + 15);
+        else if (a[0] == '-') return false;
+        else if (pos == 0) { o.kind = a; ++pos; }
+        else if (o.kind == "rmat" && pos == 1) { o.scale = atoi(a); ++pos; }
+        else if (o.kind == "rmat" && pos == 2) { o.edge_factor = atoi(a); ++pos; }
+        else if (o.kind == "grid3d" && pos == 1) { o.width = atoi(a); ++pos; }
+        else return false;
+    }
+    return o.kind == "rmat" || o.kind == "grid3d";
+}
+
+// The enactor classes are distinct types (as in the reference); run the
+// source loop once per type.
+template <typename Enactor>
+int run(const Options& o, b40c_graph* g, const std::vector<int32_t>& offs, const std::vector<int32_t>& cols) {
+    const int32_t n = (int32_t)offs.size() - 1;
+    Problem problem;
+    b40c_status s = problem.FromGraph(g);
+    if (s != B40C_OK) {
+        fprintf(stderr, "FromGraph: %s\n", b40c_last_error_string());
+        return 2;
+    }
+    Enactor enactor(false);
+    if (enactor.status() != B40C_OK) {
+        fprintf(stderr, "enactor: %s\n", b40c_last_error_string());
+        return 2;
+    }
+    std::vector<int32_t> sources;
+    if (o.src >= 0) {
+        sources.push_back(o.src);
+    } else {
+        sources.resize(o.iterations);
+        sources.resize(orc_pick_sources(n, offs.data(), o.seed, o.iterations, sources.data()));
+    }
+    std::vector<int32_t> depth(n), pred(n), ref(n);
+    int failures = 0;
+    double total_ms = 0.0;
+    long long total_edges = 0;
+    for (int32_t src : sources) {
+        s = problem.Reset(enactor.GetFrontierType(), o.queue_sizing);
+        if (s != B40C_OK) {
+            fprintf(stderr, "Reset: %s\n", b40c_last_error_string());
+            return 2;
+        }
+        const auto t0 = std::chrono::steady_clock::now();
+        if (o.mode == "fused") s = enactor.EnactFusedSearch(problem, src);
+        else if (o.mode == "iterative") s = enactor.EnactIterativeSearch(problem, src);
+        else s = enactor.EnactSearch(problem, src);
+        long long queued = 0;
+        int search_depth = 0;
+        double duty = 0.0;
+        if (s == B40C_OK) s = enactor.GetStatistics(queued, search_depth, duty);
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        if (s != B40C_OK) {
+            printf("src %d: status %d (%s)\n", src, s, b40c_last_error_string());
+            ++failures;
+            continue;
+        }
+        if (problem.ExtractResults(depth.data(), pred.data()) != B40C_OK) {
+            printf("src %d: extract failed (%s)\n", src, b40c_last_error_string());
+            ++failures;
+            continue;
+        }
+        const int32_t ref_sd = orc_bfs(n, offs.data(), cols.data(), src, ref.data(), nullptr);
+        const bool depth_ok = depth == ref;
+        const int64_t bad_pred = orc_check_pred(n, offs.data(), cols.data(), ref.data(), pred.data());
+        const bool ok = depth_ok && bad_pred < 0 && search_depth == ref_sd;
+        int64_t nodes_visited = 0;
+        const int64_t edges_visited = orc_edges_visited(n, offs.data(), ref.data(), &nodes_visited);
+        total_ms += ms;
+        total_edges += edges_visited;
+        printf("src %d: %s depth=%s pred=%s search_depth=%d (ref %d) nodes_visited=%lld edges_visited=%lld "
+               "elapsed=%.3f ms rate=%.2f MiEdges/s\n",
+               src, ok ? "CORRECT" : "INCORRECT", depth_ok ? "ok" : "MISMATCH", bad_pred < 0 ? "ok" : "BAD",
+               search_depth, ref_sd, (long long)nodes_visited, (long long)edges_visited, ms,
+               ms > 0 ? edges_visited / (ms * 1000.0) : 0.0);
+        failures += ok ? 0 : 1;
+    }
+    printf("summary: strategy=%d mode=%s sources=%zu failures=%d avg_elapsed=%.3f ms avg_rate=%.2f MiEdges/s\n",
+           o.strategy, o.mode.c_str(), sources.size(), failures, sources.empty() ? 0.0 : total_ms / sources.size(),
+           total_ms > 0 ? total_edges / (total_ms * 1000.0) : 0.0);
+    return failures ? 1 : 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    Options o;
+    if (!parse(argc, argv, o)) {
+        fprintf(stderr, "usage: %s rmat <scale> [edge_factor] | grid3d <width> [--strategy=0..5] "
+                        "[--mode=default|fused|iterative] [--i=N] [--src=V] [--seed=S] [--queue-sizing=F]\n",
+                argv[0]);
+        return 2;
+    }
+    b40c_graph* g = nullptr;
+    b40c_status s = o.kind == "rmat" ? b40c_graph_build_rmat(o.scale, o.edge_factor, 0.57, 0.19, 0.19, o.seed, &g)
+                                     : b40c_graph_build_lattice3d(o.width, o.seed, nullptr, &g);
+    if (s != B40C_OK) {
+        fprintf(stderr, "graph build: %s\n", b40c_last_error_string());
+        return 2;
+    }
+    int32_t n = 0;
+    int64_t m = 0;
+    b40c_graph_info(g, &n, &m);
+    std::vector<int32_t> offs((size_t)n + 1), cols((size_t)(m > 0 ? m : 1));
+    b40c_graph_copy_to_host(g, offs.data(), cols.data());
+    printf("graph %s: nodes=%d edges=%lld\n", o.kind.c_str(), n, (long long)m);
+    int rc = 2;
+    switch (o.strategy) {
+        case B40C_EXPAND_CONTRACT: rc = run<EnactorExpandContract<false>>(o, g, offs, cols); break;
+        case B40C_CONTRACT_EXPAND: rc = run<EnactorContractExpand<false>>(o, g, offs, cols); break;
+        case B40C_TWO_PHASE: rc = run<EnactorTwoPhase<false>>(o, g, offs, cols); break;
+        case B40C_HYBRID: rc = run<EnactorHybrid<false>>(o, g, offs, cols); break;
+        case B40C_PERSISTENT: rc = run<EnactorPersistent<false>>(o, g, offs, cols); break;
+        case B40C_DIRECTION_OPTIMIZING: rc = run<EnactorDirectionOptimizing<false>>(o, g, offs, cols); break;
+        default: fprintf(stderr, "unknown strategy %d\n", o.strategy);
+    }
+    b40c_graph_destroy(g);
+    return rc;
+}

@@ -239,3 +239,31 @@ def test_from_host_problem_signature(bfsmod, orc):
     assert prob.ExtractResults(labels) == 0
     np.testing.assert_array_equal(labels, orc.bfs(off, cols, src, with_pred=False)[0])
     assert prob.FromHostProblem(False, off.size - 1, cols.size, cols, off, 2) == b4._lib.ERR_UNSUPPORTED
+
+
+@pytest.fixture(scope="module")
+def rmat22(bfsmod, orc):
+    """The headline graph at full size (RMAT scale 22, edge factor 16,
+    a=0.57 b=0.19 c=0.19, seed 0) and the first 8 of its 64 seeded sources."""
+    g = bfsmod.Graph.rmat(22, 16, 0.57, 0.19, 0.19, 0)
+    off, cols = g.to_host()
+    srcs = [int(s) for s in orc.pick_sources(off, 1, 64)[:8]]
+    refs = orc.bfs_many(off, cols, srcs)
+    return g, off, cols, srcs, refs
+
+
+@pytest.mark.parametrize("strategy", [b4.DIRECTION_OPTIMIZING, b4.EXPAND_CONTRACT, b4.HYBRID])
+def test_rmat22_headline_sources(bfsmod, orc, rmat22, strategy):
+    """Full-size parity on the headline workload, with the tuned knobs the
+    bench uses: depth bit-exact, every predecessor valid, search depth."""
+    from back40computing_amd import autotune
+    g, off, cols, srcs, refs = rmat22
+    info = bfsmod.device_info()
+    tuning = autotune.lookup(info["arch"], strategy, g.nodes, g.edges)
+    for k, s in enumerate(srcs):
+        depth, pred, st = run(bfsmod, g, s, strategy, b4.MODE_DEFAULT, tuning=tuning)
+        ref = refs[k]
+        sd = int(ref.max()) + 1
+        np.testing.assert_array_equal(depth, ref)
+        assert orc.check_pred(off, cols, ref, pred) == -1
+        assert st["search_depth"] == sd

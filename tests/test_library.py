@@ -70,3 +70,14 @@ def test_strategy_frontier_types_match_reference_numbering():
     assert (_lib.EXPAND_CONTRACT, _lib.CONTRACT_EXPAND, _lib.TWO_PHASE, _lib.HYBRID) == (0, 1, 2, 3)
     # csr_problem.cuh:54-61
     assert (_lib.VERTEX_FRONTIERS, _lib.EDGE_FRONTIERS, _lib.MIXED_FRONTIERS) == (0, 1, 2)
+
+
+def test_cpp_driver_built_and_usage():
+    """The C++ facade compiles (the driver is built from it) and the driver
+    rejects a bad command line without touching the GPU."""
+    import os
+    import subprocess
+    drv = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "test_bfs")
+    assert os.path.exists(drv), "C++ driver not built"
+    r = subprocess.run([drv], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage:" in r.stderr
