@@ -36,7 +36,6 @@ SPACE = {
     "grid_size": [256, 512, 1024, 2048, 4096],
     "levels_per_sync": [2, 4, 8, 12, 16],
     "saturation_quit": [64, 256, 512, 2048],
-    "bottom_up_wave_threshold": [8, 16, 32, 64, 256, 4096, 1 << 30],
 }
 
 # Knobs that matter per strategy.
@@ -49,7 +48,7 @@ KNOBS = {
                      "grid_size", "levels_per_sync"],
     _lib.HYBRID: ["saturation_quit", "grid_gather_threshold", "cta_gather_threshold", "wave_gather_threshold",
                   "grid_size"],
-    _lib.DIRECTION_OPTIMIZING: ["bottom_up_wave_threshold", "do_alpha", "do_beta", "grid_gather_threshold",
+    _lib.DIRECTION_OPTIMIZING: ["do_alpha", "do_beta", "grid_gather_threshold",
                                 "grid_size", "levels_per_sync", "wave_gather_threshold"],
 }
 

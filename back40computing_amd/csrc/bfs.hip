@@ -90,15 +90,15 @@ __device__ __forceinline__ GlobalQueue out_queue(const BfsArgs& a, int ph) {
 }
 
 // Workgroup sum of a per-thread u64, added once to *dst.
-__device__ __forceinline__ void wg_add_u64(unsigned long long v, unsigned long long* dst, LevelShared& s) {
+__device__ __forceinline__ void wg_add_u64(unsigned long long v, unsigned long long* dst, unsigned long long* red) {
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, kWave);
     __syncthreads();
-    if (lane_id() == 0) s.red[threadIdx.x / kWave] = v;
+    if (lane_id() == 0) red[threadIdx.x / kWave] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
         unsigned long long t = 0;
-        for (int w = 0; w < BFS_WAVES; ++w) t += s.red[w];
+        for (int w = 0; w < BFS_WAVES; ++w) t += red[w];
         if (t) atomic_add_u64(dst, t);
     }
 }
@@ -142,7 +142,7 @@ __device__ __forceinline__ void defer_heavy(const BfsArgs& a, int ph, int32_t v,
 // and degree accounting for the direction switch, then the LDS queue.
 struct TdSink {
     const BfsArgs& a;
-    LevelShared& s;
+    LdsQueue<false>& q;
     const GlobalQueue& out;
     uint32_t* next_bm;
     int32_t depth_next;
@@ -158,7 +158,7 @@ struct TdSink {
             if (next_bm) atomic_or_u32(&next_bm[v >> 5], 1u << (v & 31));
             if (count_edges) eacc += (unsigned long long)(a.offs[v + 1] - a.offs[v]);
         }
-        lq_append(s.qv, out, won, v, 0);
+        lq_append(q, out, won, v, 0);
     }
 };
 
@@ -170,9 +170,9 @@ struct PairSink {
     }
 };
 
-__device__ __forceinline__ void td_finish(const BfsArgs& a, int ph, TdSink& sink, LevelShared& s) {
-    lq_flush(s.qv, sink.out, 1);
-    if (sink.count_edges) wg_add_u64(sink.eacc, &a.ctr->level_edges[(ph + 1) & 3], s);
+__device__ __forceinline__ void td_finish(const BfsArgs& a, int ph, TdSink& sink, unsigned long long* red) {
+    lq_flush(sink.q, sink.out, 1);
+    if (sink.count_edges) wg_add_u64(sink.eacc, &a.ctr->level_edges[(ph + 1) & 3], red);
     if (__syncthreads_or(sink.found) && threadIdx.x == 0) atomic_max_u32(a.max_depth, (uint32_t)sink.depth_next);
 }
 
@@ -181,12 +181,12 @@ __device__ __forceinline__ void td_finish(const BfsArgs& a, int ph, TdSink& sink
 // expand_contract_atomic/cta.cuh:212-214, and compacted into the next vertex
 // frontier.
 __device__ void td_phase(const BfsArgs& a, int ph, int32_t depth_next, uint32_t* next_bm, bool count_edges,
-                         LevelShared& s) {
+                         LdsQueue<false>& q, GatherSmem& g, unsigned long long* red) {
     const uint32_t n_in = phase_input_len(a, ph);
     phase_begin(a, ph, n_in);
     const int32_t* qin = a.qk[ph & 1];
     const GlobalQueue out = out_queue(a, ph);
-    TdSink sink{a, s, out, next_bm, depth_next, count_edges, false, 0ull};
+    TdSink sink{a, q, out, next_bm, depth_next, count_edges, false, 0ull};
     for (uint32_t base = blockIdx.x * BFS_THREADS; base < n_in; base += gridDim.x * BFS_THREADS) {
         const uint32_t i = base + threadIdx.x;
         int32_t u = -1, start = 0, deg = 0;
@@ -198,10 +198,10 @@ __device__ void td_phase(const BfsArgs& a, int ph, int32_t depth_next, uint32_t*
             }
         }
         defer_heavy(a, ph, u, deg);
-        tiered_gather(a.cols, u, start, deg, a.cta_th, a.wave_th, s.g, sink);
-        lq_flush(s.qv, out, LQ_CAP / 2);
+        tiered_gather(a.cols, u, start, deg, a.cta_th, a.wave_th, g, sink);
+        lq_flush(q, out, LQ_CAP / 2);
     }
-    td_finish(a, ph, sink, s);
+    td_finish(a, ph, sink, red);
 }
 
 // Contract-expand (edge frontier -> edge frontier).  Incoming (vertex,
@@ -327,67 +327,104 @@ __device__ void tp_filter_phase(const BfsArgs& a, int ph, LevelShared& s) {
 }
 
 // Bottom-up step of the direction-optimizing traversal (no counterpart in
-// the reference).  Every unvisited vertex scans its neighbours for one in the
-// current frontier bitmap and stops at the first hit.  A wave owns the two
-// bitmap words of its 64 vertices, so visited/next words are written without
-// atomics.  Discovered vertices are also appended to the vertex queue so a
-// top-down step can follow directly.  Only valid on symmetric graphs.
-__device__ void bu_phase(const BfsArgs& a, int ph, int32_t depth_next, const uint32_t* cur, uint32_t* next,
-                         LevelShared& s) {
-    phase_begin(a, ph, 0);
+// the reference): every unvisited vertex looks for a neighbour in the current
+// frontier bitmap and stops at the first hit.  Only valid on symmetric
+// graphs.  The level kernel keeps a small LDS footprint so that many waves
+// are resident: the step is a chain of dependent loads (offsets -> column ->
+// bitmap word), so latency, not bandwidth, bounds it.
+// Each workgroup takes tiles of BFS_THREADS consecutive vertices:
+//  * stage 1: each lane checks the first BU_FIRST neighbours of its vertex
+//    with independent loads;
+//  * stage 2: vertices still unresolved with neighbours left are scanned by
+//    BU_GROUP-lane groups, BU_GROUP neighbours per step, lowest lane wins
+//    (the first frontier neighbour in list order).
+// A wave owns the two bitmap words of its 64 vertices, so visited/next words
+// are written without atomics.  Discovered vertices are appended to the
+// vertex queue so that a top-down step can follow directly.
+constexpr int BU_FIRST = 4;
+constexpr int BU_GROUP = 8;
+constexpr int BU_GROUPS = BFS_THREADS / BU_GROUP;
+
+struct BuShared {
+    LdsQueue<false> q;
+    int32_t pv[BFS_THREADS];  // unresolved vertex (tile-local index)
+    int32_t pe[BFS_THREADS];  // next edge to scan
+    int32_t pend_end[BFS_THREADS];
+    int32_t parent[BFS_THREADS];
+    unsigned long long red[BFS_WAVES];
+    uint32_t npend;
+};
+
+// LDS of a direction-optimizing level: the top-down step's queue and gather
+// staging or the bottom-up step's tile state (about 16.5 KB: eight
+// workgroups of 256 threads stay resident per CU).
+struct DoTdShared {
+    LdsQueue<false> qv;
+    GatherSmem g;
+    unsigned long long red[BFS_WAVES];
+};
+union DoShared {
+    DoTdShared td;
+    BuShared bu;
+};
+
+__device__ __forceinline__ bool bm_test(const uint32_t* bm, int32_t u) { return (bm[u >> 5] >> (u & 31)) & 1u; }
+
+__device__ void bu_tiles(const BfsArgs& a, int ph, int32_t depth_next, const uint32_t* cur, uint32_t* next,
+                         BuShared& s) {
     const GlobalQueue out = out_queue(a, ph);
+    const int tid = threadIdx.x;
     const int lane = lane_id();
-    const int wid = threadIdx.x / kWave;
-    const int64_t wave0 = (int64_t)blockIdx.x * BFS_WAVES + wid;
-    const int64_t nwaves = (int64_t)gridDim.x * BFS_WAVES;
+    const int grp = tid / BU_GROUP, sub = tid % BU_GROUP;
     bool found_any = false;
     unsigned long long eacc = 0;
-    for (int64_t vb = wave0 * kWave; vb < a.n; vb += nwaves * kWave) {
-        const int32_t v = (int32_t)(vb + lane);
+    for (int64_t base = (int64_t)blockIdx.x * BFS_THREADS; base < a.n; base += (int64_t)gridDim.x * BFS_THREADS) {
+        if (tid == 0) s.npend = 0;
+        const int32_t v = (int32_t)(base + tid);
         const uint32_t vw = (v < a.n) ? a.visited[v >> 5] : 0xFFFFFFFFu;
         const bool active = (v < a.n) && !((vw >> (v & 31)) & 1u);
-        int32_t parent = -1;
-        int32_t deg = 0, start = 0;
+        int32_t start = 0, deg = 0, parent = -1;
         if (active) {
             start = a.offs[v];
             deg = a.offs[v + 1] - start;
         }
-        // Long lists: the whole wave scans one list 64 neighbours at a time and
-        // stops at the first chunk with a frontier neighbour (lowest lane wins,
-        // i.e. the first such neighbour in list order).
-        uint64_t lm = __ballot(active && deg > a.bu_wave_th);
-        while (lm) {
-            const int leader = __ffsll((unsigned long long)lm) - 1;
-            lm &= lm - 1;
-            const int32_t ls = wave_bcast(start, leader), ld = wave_bcast(deg, leader);
+        __syncthreads();  // npend reset visible; previous tile's stage 2 done
+        // Stage 1: the first BU_FIRST neighbours, loads issued together.
+        if (deg > 0) {
+            int32_t u[BU_FIRST];
+#pragma unroll
+            for (int k = 0; k < BU_FIRST; ++k) u[k] = (k < deg) ? a.cols[start + k] : -1;
+#pragma unroll
+            for (int k = BU_FIRST - 1; k >= 0; --k)
+                if (u[k] >= 0 && bm_test(cur, u[k])) parent = u[k];
+            if (parent < 0 && deg > BU_FIRST) {
+                const uint32_t slot = atomicAdd(&s.npend, 1u);
+                s.pv[slot] = tid;
+                s.pe[slot] = start + BU_FIRST;
+                s.pend_end[slot] = start + deg;
+            }
+        }
+        s.parent[tid] = parent;
+        __syncthreads();
+        // Stage 2: groups of BU_GROUP lanes scan the unresolved lists.
+        const uint32_t np = s.npend;
+        for (uint32_t i = grp; i < np; i += BU_GROUPS) {
+            const int32_t e1 = s.pend_end[i];
             int32_t hit_u = -1;
-            for (int32_t e0 = 0; e0 < ld; e0 += kWave) {
-                const int32_t e = e0 + lane;
-                const int32_t u = (e < ld) ? a.cols[ls + e] : -1;
-                const bool hit = u >= 0 && ((cur[u >> 5] >> (u & 31)) & 1u);
-                const uint64_t hm = __ballot(hit);
+            for (int32_t e0 = s.pe[i]; e0 < e1; e0 += BU_GROUP) {
+                const int32_t e = e0 + sub;
+                const int32_t u = (e < e1) ? a.cols[e] : -1;
+                const bool hit = u >= 0 && bm_test(cur, u);
+                const uint64_t hm = (__ballot(hit) >> (lane & ~(BU_GROUP - 1))) & ((1ull << BU_GROUP) - 1);
                 if (hm) {
-                    hit_u = wave_bcast(u, __ffsll((unsigned long long)hm) - 1);
+                    hit_u = __shfl(u, (lane & ~(BU_GROUP - 1)) + __ffsll((unsigned long long)hm) - 1, kWave);
                     break;
                 }
             }
-            if (lane == leader) parent = hit_u;
+            if (sub == 0 && hit_u >= 0) s.parent[s.pv[i]] = hit_u;
         }
-        // Short lists: each lane scans its own list, four neighbours per step.
-        if (active && deg <= a.bu_wave_th) {
-            int32_t e = start;
-            const int32_t end = start + deg;
-            for (; e + 4 <= end && parent < 0; e += 4) {
-                const int32_t u0 = a.cols[e], u1 = a.cols[e + 1], u2 = a.cols[e + 2], u3 = a.cols[e + 3];
-                const uint32_t b0 = (cur[u0 >> 5] >> (u0 & 31)) & 1u, b1 = (cur[u1 >> 5] >> (u1 & 31)) & 1u;
-                const uint32_t b2 = (cur[u2 >> 5] >> (u2 & 31)) & 1u, b3 = (cur[u3 >> 5] >> (u3 & 31)) & 1u;
-                parent = b0 ? u0 : b1 ? u1 : b2 ? u2 : b3 ? u3 : -1;
-            }
-            for (; e < end && parent < 0; ++e) {
-                const int32_t u = a.cols[e];
-                if ((cur[u >> 5] >> (u & 31)) & 1u) parent = u;
-            }
-        }
+        __syncthreads();
+        parent = s.parent[tid];
         const bool found = parent >= 0;
         if (found) {
             a.depth[v] = depth_next;
@@ -397,18 +434,28 @@ __device__ void bu_phase(const BfsArgs& a, int ph, int32_t depth_next, const uin
         }
         const uint64_t fm = __ballot(found);
         if ((lane & 31) == 0) {
-            const int64_t w = (vb >> 5) + (lane >> 5);
+            const int64_t w = ((base + tid) >> 5);
             if (w < a.words) {
                 const uint32_t bits = (lane == 0) ? (uint32_t)fm : (uint32_t)(fm >> 32);
                 next[w] = bits;
                 if (bits) a.visited[w] = vw | bits;
             }
         }
-        lq_append(s.qv, out, found, v, 0);
+        lq_append(s.q, out, found, v, 0);
+        lq_flush(s.q, out, LQ_CAP / 2);
     }
-    lq_flush(s.qv, out, 1);
-    wg_add_u64(eacc, &a.ctr->level_edges[(ph + 1) & 3], s);
-    if (__syncthreads_or(found_any) && threadIdx.x == 0) atomic_max_u32(a.max_depth, (uint32_t)depth_next);
+    lq_flush(s.q, out, 1);
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) eacc += __shfl_xor(eacc, d, kWave);
+    __syncthreads();
+    if (lane == 0) s.red[tid / kWave] = eacc;
+    __syncthreads();
+    if (tid == 0) {
+        unsigned long long t = 0;
+        for (int w = 0; w < BFS_WAVES; ++w) t += s.red[w];
+        if (t) atomic_add_u64(&a.ctr->level_edges[(ph + 1) & 3], t);
+    }
+    if (__syncthreads_or(found_any) && tid == 0) atomic_max_u32(a.max_depth, (uint32_t)depth_next);
 }
 
 // Direction for level ph of the direction-optimizing traversal (Beamer,
@@ -528,7 +575,7 @@ __global__ __launch_bounds__(BFS_THREADS) void k_phase(BfsArgs a, int ph, int32_
     unsigned long long t0 = 0;
     stats_begin(t0, a);
     shared_init(s);
-    if (KIND == PK_TD) td_phase(a, ph, depth, nullptr, false, s);
+    if (KIND == PK_TD) td_phase(a, ph, depth, nullptr, false, s.qv, s.g, s.red);
     if (KIND == PK_CE) ce_phase(a, ph, depth, s);
     if (KIND == PK_TP_CONTRACT) tp_contract_phase(a, ph, depth, s);
     if (KIND == PK_TP_EXPAND) tp_expand_phase(a, ph, s);
@@ -541,7 +588,7 @@ __global__ __launch_bounds__(BFS_THREADS) void k_phase(BfsArgs a, int ph, int32_
 // run a top-down or bottom-up step.  An empty frontier makes it a no-op, so
 // the host can launch levels ahead of the termination check.
 __global__ __launch_bounds__(BFS_THREADS) void k_do_level(BfsArgs a, int ph, int slot) {
-    __shared__ LevelShared s;
+    __shared__ DoShared s;
     int32_t unused_depth = 0;
     resolve_phase(a, ph, unused_depth);
     const uint32_t n_f = phase_input_len(a, ph);
@@ -559,11 +606,17 @@ __global__ __launch_bounds__(BFS_THREADS) void k_do_level(BfsArgs a, int ph, int
     }
     uint32_t* clear = a.fbm[(ph + 2) % 3];
     for (int32_t w = blockIdx.x * BFS_THREADS + threadIdx.x; w < a.words; w += gridDim.x * BFS_THREADS) clear[w] = 0;
-    shared_init(s);
-    if (dir == 0)
-        td_phase(a, ph, ph + 1, a.fbm[(ph + 1) % 3], true, s);
-    else
-        bu_phase(a, ph, ph + 1, a.fbm[ph % 3], a.fbm[(ph + 1) % 3], s);
+    if (dir == 0) {
+        if (threadIdx.x == 0) s.td.qv.count = 0;
+        gather_init(s.td.g);
+        __syncthreads();
+        td_phase(a, ph, ph + 1, a.fbm[(ph + 1) % 3], true, s.td.qv, s.td.g, s.td.red);
+    } else {
+        phase_begin(a, ph, 0);
+        if (threadIdx.x == 0) s.bu.q.count = 0;
+        __syncthreads();
+        bu_tiles(a, ph, ph + 1, a.fbm[ph % 3], a.fbm[(ph + 1) % 3], s.bu);
+    }
     stats_end(t0, a, slot);
 }
 
@@ -573,9 +626,24 @@ __global__ __launch_bounds__(BFS_THREADS) void k_do_level(BfsArgs a, int ph, int
 // entry it names, so stale map words can never index outside a list.
 enum HeavyKind { HK_TD = 0, HK_TD_DO = 1, HK_PAIRS = 2 };
 
+// Top-down grid-tier launches need only a vertex queue; a small LDS block
+// keeps them at full residency.
+struct TdHeavyShared {
+    LdsQueue<false> qv;
+    unsigned long long red[BFS_WAVES];
+};
+template <int KIND>
+struct HeavySharedOf {
+    typedef TdHeavyShared type;
+};
+template <>
+struct HeavySharedOf<HK_PAIRS> {
+    typedef LevelShared type;
+};
+
 template <int KIND>
 __global__ __launch_bounds__(BFS_THREADS) void k_heavy(BfsArgs a, int ph, int32_t depth_next, int slot) {
-    __shared__ LevelShared s;
+    __shared__ typename HeavySharedOf<KIND>::type s;
     resolve_phase(a, ph, depth_next);
     const unsigned long long alloc =
         __hip_atomic_load(&a.ctr->heavy_alloc[ph & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -584,15 +652,18 @@ __global__ __launch_bounds__(BFS_THREADS) void k_heavy(BfsArgs a, int ph, int32_
     if (H == 0 || C == 0) return;
     unsigned long long t0 = 0;
     stats_begin(t0, a);
-    shared_init(s);
+    if constexpr (KIND == HK_PAIRS) {
+        shared_init(s);
+    } else {
+        if (threadIdx.x == 0) s.qv.count = 0;
+        __syncthreads();
+    }
     const GlobalQueue out = out_queue(a, ph);
-    TdSink tds{a, s, out, (KIND == HK_TD_DO) ? a.fbm[(ph + 1) % 3] : nullptr, depth_next, KIND == HK_TD_DO,
-               false, 0ull};
-    PairSink ps{s, out};
     int it = 0;
-    for (uint32_t c = blockIdx.x; c < C; c += gridDim.x) {
+    auto chunk = [&](uint32_t c, int32_t& u, int32_t& v, bool& valid) {
         const int32_t entry = a.chunk_map[c];
-        int32_t u = -1, start = 0, len = 0;
+        int32_t start = 0, len = 0;
+        u = -1;
         if ((uint32_t)entry < H) {
             const int2 eu = a.heavy[entry];
             if (valid_vertex(a, eu.x) && (uint32_t)eu.y <= c) {
@@ -606,23 +677,31 @@ __global__ __launch_bounds__(BFS_THREADS) void k_heavy(BfsArgs a, int ph, int32_
             }
         }
         const int32_t k = (int32_t)threadIdx.x;
-        const bool valid = k < len;
-        const int32_t v = valid ? a.cols[start + k] : 0;
-        if (KIND == HK_PAIRS)
+        valid = k < len;
+        v = valid ? a.cols[start + k] : 0;
+    };
+    if constexpr (KIND == HK_PAIRS) {
+        PairSink ps{s, out};
+        for (uint32_t c = blockIdx.x; c < C; c += gridDim.x) {
+            int32_t u, v;
+            bool valid;
+            chunk(c, u, v, valid);
             ps(valid, v, u);
-        else
-            tds(valid, v, u);
-        if ((++it & 3) == 0) {
-            if (KIND == HK_PAIRS)
-                lq_flush(s.qp, out, LQ_CAP / 2);
-            else
-                lq_flush(s.qv, out, LQ_CAP / 2);
+            if ((++it & 3) == 0) lq_flush(s.qp, out, LQ_CAP / 2);
         }
-    }
-    if (KIND == HK_PAIRS)
         lq_flush(s.qp, out, 1);
-    else
-        td_finish(a, ph, tds, s);
+    } else {
+        TdSink tds{a, s.qv, out, (KIND == HK_TD_DO) ? a.fbm[(ph + 1) % 3] : nullptr, depth_next, KIND == HK_TD_DO,
+                   false, 0ull};
+        for (uint32_t c = blockIdx.x; c < C; c += gridDim.x) {
+            int32_t u, v;
+            bool valid;
+            chunk(c, u, v, valid);
+            tds(valid, v, u);
+            if ((++it & 3) == 0) lq_flush(s.qv, out, LQ_CAP / 2);
+        }
+        td_finish(a, ph, tds, s.red);
+    }
     stats_end(t0, a, slot);
 }
 
@@ -655,7 +734,7 @@ __global__ __launch_bounds__(BFS_THREADS) void k_fused(BfsArgs a, int ph, int32_
         unsigned long long tb = 0;
         if (a.wg_stats && threadIdx.x == 0) tb = wg_clock();
         if (KIND == FK_TD) {
-            td_phase(a, ph, depth + 1, nullptr, false, s);
+            td_phase(a, ph, depth + 1, nullptr, false, s.qv, s.g, s.red);
             ph += 1;
         } else if (KIND == FK_CE) {
             ce_phase(a, ph, depth, s);

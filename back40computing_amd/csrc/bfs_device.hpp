@@ -57,7 +57,6 @@ struct BfsArgs {
     int32_t do_alpha;
     int32_t do_beta;
     int32_t dev_base;        // 1: phase = ctr->ph_base + kernel argument (captured, replayed level loops)
-    int32_t bu_wave_th;      // bottom-up: lists longer than this are scanned by the whole wave
     long long edges;
 };
 

@@ -23,11 +23,12 @@
 //    load-balanced grid tier.
 //  * k_ms_pull: bottom-up pull over the vertices of graphs with bits in
 //    bottom-up mode (each vertex scans its neighbours until every wanted bit
-//    is found; long lists are scanned by a whole wave).  A young query's bit
-//    is never in a bottom-up want set, so the pull's early exit keeps
-//    working while queries of every age share it.
+//    is found; unresolved lists are scanned by 8-lane groups).  A young
+//    query's bit is never in a bottom-up want set, so the pull's early exit
+//    keeps working while queries of every age share it.
 //  * k_ms_count: per (graph, slot) count and degree sum of the vertices
-//    discovered, by a 64x64 bit transpose across each wave.
+//    discovered, by a 64x64 bit transpose across each wave; also the next
+//    level's one-bit-per-vertex frontier bitmap.
 //  * k_ms_direction: per slot, Beamer's rule (Beamer, Asanovic, Patterson,
 //    SC 2012) on the slot's own counts -> next level's bottom-up masks, and
 //    the per-graph "found anything" mask the host uses to retire queries.
@@ -73,6 +74,7 @@ struct MsArgs {
     int32_t* depth_out;
     int32_t* pred_out;
     int32_t* queue[2];
+    uint32_t* fbm;                     // [ceil(n/32)] vertices carrying any bit in the current masks
     int2* heavy;                       // grid-tier entries (vertex, first chunk)
     int32_t* chunk_map;                // chunk -> entry
     uint32_t qcap;
@@ -86,7 +88,6 @@ struct MsArgs {
     int32_t cta_th;
     int32_t wave_th;
     int32_t grid_th;
-    int32_t bu_wave_th;
     int32_t dev_base;
 };
 
@@ -95,24 +96,6 @@ __device__ __forceinline__ unsigned long long atomic_or_u64(unsigned long long* 
 }
 __device__ __forceinline__ void atomic_and_u64(unsigned long long* p, unsigned long long v) {
     (void)__hip_atomic_fetch_and(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ __forceinline__ unsigned long long wave_or64(unsigned long long x) {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) x |= __shfl_xor(x, d, kWave);
-    return x;
-}
-
-// OR of x over the lanes below this one.
-__device__ __forceinline__ unsigned long long wave_excl_or64(unsigned long long x) {
-    const int lane = lane_id();
-#pragma unroll
-    for (int d = 1; d < kWave; d <<= 1) {
-        const unsigned long long y = __shfl_up(x, d, kWave);
-        if (lane >= d) x |= y;
-    }
-    const unsigned long long e = __shfl_up(x, 1, kWave);
-    return lane == 0 ? 0ull : e;
 }
 
 // 64x64 bit-matrix transpose across a wave: on return, bit i of lane b's
@@ -270,10 +253,47 @@ __global__ __launch_bounds__(BFS_THREADS) void k_ms_push(MsArgs a, int ph) {
 // Bottom-up pull of level ph for the bits in bottom-up mode: each vertex of
 // such a graph scans its neighbours until every wanted bit is found.  Bits
 // are disjoint from the push's, but both OR into visited[v] and next[v], so
-// both use atomics there.
+// both use atomics there.  A neighbour's 64-bit mask is loaded only when the
+// frontier bitmap (one bit per vertex: "carries any bit this level", built
+// by k_ms_count, small enough to stay in L2) says it is in some frontier.
+// Workgroups take tiles of BFS_THREADS consecutive vertices:
+//  * stage 1: each lane checks the first MS_FIRST neighbours of its vertex;
+//  * stage 2: vertices still missing wanted bits are scanned by MS_GROUP-lane
+//    groups, MS_GROUP neighbours per step; each new bit is credited to the
+//    first neighbour (in list order) that carries it.
+constexpr int MS_FIRST = 4;
+constexpr int MS_GROUP = 8;
+constexpr int MS_GROUPS = BFS_THREADS / MS_GROUP;
+
 struct MsPullShared {
     LdsQueue<false> q;
+    int32_t pv[BFS_THREADS];  // unresolved vertex (tile-local index)
+    int32_t pe[BFS_THREADS];  // next edge to scan
+    int32_t pend_end[BFS_THREADS];
+    unsigned long long pwant[BFS_THREADS];
+    unsigned long long got[BFS_THREADS];
+    uint32_t npend;
 };
+
+__device__ __forceinline__ bool ms_bm_test(const uint32_t* bm, int32_t u) { return (bm[u >> 5] >> (u & 31)) & 1u; }
+
+// OR over the MS_GROUP lanes of this lane's group, and over the group's
+// lanes below this one.
+__device__ __forceinline__ unsigned long long group_or(unsigned long long x) {
+#pragma unroll
+    for (int d = MS_GROUP / 2; d >= 1; d >>= 1) x |= __shfl_xor(x, d, kWave);
+    return x;
+}
+__device__ __forceinline__ unsigned long long group_excl_or(unsigned long long x) {
+    const int sub = lane_id() & (MS_GROUP - 1);
+#pragma unroll
+    for (int d = 1; d < MS_GROUP; d <<= 1) {
+        const unsigned long long y = __shfl_up(x, d, kWave);
+        if (sub >= d) x |= y;
+    }
+    const unsigned long long e = __shfl_up(x, 1, kWave);
+    return sub == 0 ? 0ull : e;
+}
 
 __global__ __launch_bounds__(BFS_THREADS) void k_ms_pull(MsArgs a, int ph) {
     __shared__ MsPullShared s;
@@ -286,93 +306,82 @@ __global__ __launch_bounds__(BFS_THREADS) void k_ms_pull(MsArgs a, int ph) {
     const GlobalQueue out = ms_out_queue(a, ph);
     const unsigned long long* cur = a.front[ph % 3];
     unsigned long long* next = a.front[(ph + 1) % 3];
-    {
-        const int lane = lane_id();
-        const int wid = threadIdx.x / kWave;
-        const int64_t nwaves = (int64_t)gridDim.x * BFS_WAVES;
-        for (int64_t vb = ((int64_t)blockIdx.x * BFS_WAVES + wid) * kWave; vb < a.n; vb += nwaves * kWave) {
-            const int32_t v = (int32_t)(vb + lane);
-            unsigned long long want = 0, got = 0;
-            int32_t start = 0, deg = 0;
-            if (v < a.n) {
-                const unsigned long long m = a.bu[a.gid[v]];
-                if (m) {
-                    want = m & ~a.visited[v];
-                    if (want) {
-                        start = a.offs[v];
-                        deg = a.offs[v + 1] - start;
-                    }
+    const uint32_t* fbm = a.fbm;
+    const int tid = threadIdx.x;
+    const int lane = lane_id();
+    const int grp = tid / MS_GROUP, sub = tid % MS_GROUP;
+    for (int64_t base = (int64_t)blockIdx.x * BFS_THREADS; base < a.n; base += (int64_t)gridDim.x * BFS_THREADS) {
+        if (tid == 0) s.npend = 0;
+        const int32_t v = (int32_t)(base + tid);
+        unsigned long long want = 0, got = 0;
+        int32_t start = 0, deg = 0;
+        if (v < a.n) {
+            const unsigned long long m = a.bu[a.gid[v]];
+            if (m) {
+                want = m & ~a.visited[v];
+                if (want) {
+                    start = a.offs[v];
+                    deg = a.offs[v + 1] - start;
                 }
             }
-            // Long lists: the whole wave scans one list 64 neighbours at a
-            // time; each newly found bit is recorded by the lowest lane that
-            // holds it (the first such neighbour in list order).
-            uint64_t lm = __ballot(want != 0ull && deg > a.bu_wave_th);
-            while (lm) {
-                const int leader = __ffsll((unsigned long long)lm) - 1;
-                lm &= lm - 1;
-                const int32_t ls = wave_bcast(start, leader), ld = wave_bcast(deg, leader);
-                const unsigned long long lwant = __shfl(want, leader, kWave);
-                const int32_t lv = (int32_t)vb + leader;
-                unsigned long long lgot = 0;
-                for (int32_t e0 = 0; e0 < ld && lgot != lwant; e0 += kWave) {
-                    const int32_t e = e0 + lane;
-                    int32_t u = -1;
-                    unsigned long long f = 0;
-                    if (e < ld) {
-                        u = a.cols[ls + e];
-                        f = cur[u] & lwant & ~lgot;
-                    }
-                    const unsigned long long mine = f & ~wave_excl_or64(f);
-                    if (mine) ms_record(a, ph, lv, u, mine);
-                    lgot |= wave_or64(f);
-                }
-                if (lane == leader) got = lgot;
-            }
-            // Short lists: each lane scans its own, four neighbours per step.
-            if (want != 0ull && deg <= a.bu_wave_th) {
-                int32_t e = start;
-                const int32_t end = start + deg;
-                for (; e + 4 <= end && got != want; e += 4) {
-                    const int32_t u0 = a.cols[e], u1 = a.cols[e + 1], u2 = a.cols[e + 2], u3 = a.cols[e + 3];
-                    const unsigned long long c0 = cur[u0], c1 = cur[u1], c2 = cur[u2], c3 = cur[u3];
-                    unsigned long long f = c0 & want & ~got;
-                    if (f) {
-                        got |= f;
-                        ms_record(a, ph, v, u0, f);
-                    }
-                    f = c1 & want & ~got;
-                    if (f) {
-                        got |= f;
-                        ms_record(a, ph, v, u1, f);
-                    }
-                    f = c2 & want & ~got;
-                    if (f) {
-                        got |= f;
-                        ms_record(a, ph, v, u2, f);
-                    }
-                    f = c3 & want & ~got;
-                    if (f) {
-                        got |= f;
-                        ms_record(a, ph, v, u3, f);
-                    }
-                }
-                for (; e < end && got != want; ++e) {
-                    const int32_t u = a.cols[e];
-                    const unsigned long long f = cur[u] & want & ~got;
-                    if (f) {
-                        got |= f;
-                        ms_record(a, ph, v, u, f);
-                    }
-                }
-            }
-            bool enq = false;
-            if (got) {
-                atomic_or_u64(&a.visited[v], got);
-                enq = atomic_or_u64(&next[v], got) == 0ull;
-            }
-            lq_append(s.q, out, enq, v, 0);
         }
+        __syncthreads();  // npend reset visible; the previous tile is done with the LDS arrays
+        // Stage 1: the first MS_FIRST neighbours, loads issued together.
+        if (want != 0ull && deg > 0) {
+            int32_t u[MS_FIRST];
+#pragma unroll
+            for (int k = 0; k < MS_FIRST; ++k) u[k] = (k < deg) ? a.cols[start + k] : -1;
+#pragma unroll
+            for (int k = 0; k < MS_FIRST; ++k) {
+                if (u[k] >= 0 && ms_bm_test(fbm, u[k])) {
+                    const unsigned long long f = cur[u[k]] & want & ~got;
+                    if (f) {
+                        got |= f;
+                        ms_record(a, ph, v, u[k], f);
+                    }
+                }
+            }
+            if (got != want && deg > MS_FIRST) {
+                const uint32_t slot = atomicAdd(&s.npend, 1u);
+                s.pv[slot] = tid;
+                s.pe[slot] = start + MS_FIRST;
+                s.pend_end[slot] = start + deg;
+                s.pwant[slot] = want;
+            }
+        }
+        s.got[tid] = got;
+        __syncthreads();
+        // Stage 2: groups of MS_GROUP lanes scan the remaining lists.
+        const uint32_t np = s.npend;
+        for (uint32_t i = grp; i < np; i += MS_GROUPS) {
+            const int32_t t = s.pv[i];
+            const int32_t vv = (int32_t)base + t;
+            const int32_t e1 = s.pend_end[i];
+            const unsigned long long w = s.pwant[i];
+            unsigned long long g = s.got[t];
+            for (int32_t e0 = s.pe[i]; e0 < e1 && g != w; e0 += MS_GROUP) {
+                const int32_t e = e0 + sub;
+                int32_t u = -1;
+                unsigned long long f = 0;
+                if (e < e1) {
+                    u = a.cols[e];
+                    if (ms_bm_test(fbm, u)) f = cur[u] & w & ~g;
+                }
+                const unsigned long long mine = f & ~group_excl_or(f);
+                if (mine) ms_record(a, ph, vv, u, mine);
+                g |= group_or(f);
+            }
+            if (sub == 0) s.got[t] = g;
+        }
+        __syncthreads();
+        got = s.got[tid];
+        bool enq = false;
+        if (got) {
+            atomic_or_u64(&a.visited[v], got);
+            enq = atomic_or_u64(&next[v], got) == 0ull;
+        }
+        lq_append(s.q, out, enq, v, 0);
+        lq_flush(s.q, out, LQ_CAP / 2);
     }
     lq_flush(s.q, out, 1);
 }
@@ -478,6 +487,17 @@ __global__ __launch_bounds__(BFS_THREADS) void k_ms_count(MsArgs a, int ph) {
     if (accg >= 0 && accn) {
         atomic_add_u32(&a.cnt_n[(size_t)accg * 64 + lane], accn);
         atomic_add_u64(&a.cnt_m[(size_t)accg * 64 + lane], accm);
+    }
+    // Frontier bitmap of the next level: bit v = (next[v] != 0).  Each wave
+    // writes the two words of its 64 vertices.
+    const int64_t words = ((int64_t)a.n + 31) / 32;
+    for (int64_t vb = wave * kWave; vb < a.n; vb += nwaves * kWave) {
+        const int64_t v = vb + lane;
+        const uint64_t m = __ballot(v < a.n && nxt[v] != 0ull);
+        if ((lane & 31) == 0) {
+            const int64_t w = v >> 5;
+            if (w < words) a.fbm[w] = (lane == 0) ? (uint32_t)m : (uint32_t)(m >> 32);
+        }
     }
 }
 
@@ -679,7 +699,7 @@ struct b40c_msbfs {
     int64_t arena_elems = 0;  // per label array (depth and pred each)
     b40c::DeviceBuffer visited, front[3], active, bu, newbits, cnt_n, cnt_m, nf_prev, m_seen, bitq, lbase, qstart, depth,
         pred, queue[2], counters;
-    b40c::DeviceBuffer seed_buf, heavy, chunk_map;
+    b40c::DeviceBuffer seed_buf, heavy, chunk_map, fbm;
     b40c::MsSeed* h_seeds = nullptr;  // pinned staging
     int h_seed_cap = 0;
     uint32_t qcap = 0, heavy_cap = 0, chunk_cap = 0;
@@ -723,6 +743,7 @@ MsArgs ms_args(b40c_msbfs* m) {
     a.queue[0] = m->queue[0].as<int32_t>();
     a.queue[1] = m->queue[1].as<int32_t>();
     a.qcap = m->qcap;
+    a.fbm = m->fbm.as<uint32_t>();
     a.heavy = m->heavy.as<int2>();
     a.chunk_map = m->chunk_map.as<int32_t>();
     a.heavy_cap = m->heavy_cap;
@@ -735,7 +756,6 @@ MsArgs ms_args(b40c_msbfs* m) {
     a.cta_th = 256;
     a.wave_th = 64;
     a.grid_th = 128;
-    a.bu_wave_th = 32;
     a.dev_base = 0;
     return a;
 }
@@ -945,6 +965,7 @@ b40c_status b40c_msbfs_create(b40c_multigraph* mg, int64_t label_budget_bytes, i
     m->chunk_cap = (uint32_t)std::min<int64_t>(mg->packed->edges / HEAVY_CHUNK + (int64_t)m->n + 64, 0x7FFFFFFFll);
     if (s == B40C_OK) s = m->heavy.alloc((size_t)m->heavy_cap * sizeof(int2));
     if (s == B40C_OK) s = m->chunk_map.alloc((size_t)m->chunk_cap * 4);
+    if (s == B40C_OK) s = m->fbm.alloc(((size_t)m->n + 31) / 32 * 4 + 4);
     if (s != B40C_OK) {
         b40c_msbfs_destroy(m);
         return s;
@@ -956,6 +977,7 @@ b40c_status b40c_msbfs_create(b40c_multigraph* mg, int64_t label_budget_bytes, i
     for (int i = 0; i < 3; ++i) B40C_HIP(hipMemsetAsync(m->front[i].ptr, 0, m->front[i].bytes, m->stream));
     B40C_HIP(hipMemsetAsync(m->newbits.ptr, 0, m->newbits.bytes, m->stream));
     B40C_HIP(hipMemsetAsync(m->bu.ptr, 0, m->bu.bytes, m->stream));
+    B40C_HIP(hipMemsetAsync(m->fbm.ptr, 0, m->fbm.bytes, m->stream));
     B40C_HIP(hipMemsetAsync(m->cnt_n.ptr, 0, m->cnt_n.bytes, m->stream));
     B40C_HIP(hipMemsetAsync(m->cnt_m.ptr, 0, m->cnt_m.bytes, m->stream));
     B40C_HIP(hipMemsetAsync(m->nf_prev.ptr, 0, m->nf_prev.bytes, m->stream));
@@ -1085,6 +1107,7 @@ void b40c_msbfs_destroy(b40c_msbfs* m) {
     m->queue[1].release();
     m->heavy.release();
     m->chunk_map.release();
+    m->fbm.release();
     m->counters.release();
     m->seed_buf.release();
     if (m->h_seeds) (void)hipHostFree(m->h_seeds);

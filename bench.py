@@ -124,6 +124,37 @@ def pct(x, q):
     return float(np.percentile(np.asarray(x), q)) if len(x) else None
 
 
+def bench_lattice(bfs, b4, hip, info, width=200, sources=4, steps=8):
+    """High-diameter workload: 200^3 6-neighbour lattice, ids permuted from
+    seed 0, `sources` seeded sources; GTEPS per strategy (one launch per
+    level, levels captured in hipGraphs where the strategy allows)."""
+    from back40computing_amd.sources import pick_sources
+    g = bfs.Graph.lattice3d(width, 0)
+    off = np.empty(g.nodes + 1, np.int32)
+    b4._lib.check(b4.lib().b40c_graph_copy_to_host(g.handle, b4._lib.ptr(off), None))
+    deg = np.diff(off).astype(np.int64)
+    srcs = pick_sources(off, 3, sources)
+    out = {"workload": "lattice%d^3_perm_seed0" % width, "vertices": g.nodes, "csr_edges": g.edges,
+           "sources": len(srcs)}
+    for s_id, name in ((b4.EXPAND_CONTRACT, "expand_contract"), (b4.HYBRID, "hybrid"),
+                       (b4.DIRECTION_OPTIMIZING, "direction_optimizing")):
+        t = tuning_for(s_id, g, info["arch"])
+        e_trav, v_vis, levels = [], [], []
+        for s in srcs:
+            depth, _, st = bfs.bfs(g, int(s), strategy=s_id, tuning=t)
+            m = depth >= 0
+            e_trav.append(int(deg[m].sum()))
+            v_vis.append(int(m.sum()))
+            levels.append(st["search_depth"])
+        r = measure_strategy(bfs, b4, hip, g, srcs, e_trav, v_vis, s_id, steps, 1, tuning=t)
+        out[name] = {"gteps": round(r["gteps"], 3), "ms_per_search": round(r["ms_per_search_enact"], 3),
+                     "levels_mean": round(float(np.mean(levels)), 1),
+                     "roofline_frac": round(r["achieved_gbs"] / HBM_PEAK_GBS, 4)}
+        log("lattice %s: %s" % (name, out[name]))
+    g.close()
+    return out
+
+
 def bench_batched(b4, info):
     """1,024 queries over 256 RMAT graphs (scale integer-uniform in [10,18],
     4 sources each): the batched multi-source engine vs the plain
@@ -275,6 +306,46 @@ def pick_sources_cached(off, seed, count):
     return pick_sources(off, seed, count)
 
 
+def probe(args):
+    """Short run for hardware-counter passes (rocprofv3 --pmc): a 1 GiB
+    device-to-device copy, whose FETCH_SIZE / WRITE_SIZE readings scale the
+    counters to bytes, then `--steps` headline searches (Reset + EnactSearch)
+    on the RMAT graph with the tuned knobs."""
+    import back40computing_amd as b4
+    from back40computing_amd import bfs
+    from back40computing_amd.sources import pick_sources
+    hip = Hip()
+    hip.L.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    nbytes = 1 << 30
+    bufs = [ctypes.c_void_p(), ctypes.c_void_p()]
+    for bptr in bufs:
+        b4._lib.check(b4.lib().b40c_device_alloc(ctypes.byref(bptr), nbytes))
+    assert hip.L.hipMemcpy(bufs[1], bufs[0], nbytes, 3) == 0  # hipMemcpyDeviceToDevice
+    hip.sync()
+    log("probe: copied %d bytes device to device" % nbytes)
+    for bptr in bufs:
+        b4._lib.check(b4.lib().b40c_device_free(bptr))
+    info = bfs.device_info()
+    graph = bfs.Graph.rmat(args.scale, 16, 0.57, 0.19, 0.19, 0)
+    off = np.empty(graph.nodes + 1, np.int32)
+    b4._lib.check(b4.lib().b40c_graph_copy_to_host(graph.handle, b4._lib.ptr(off), None))
+    srcs = pick_sources(off, 1, args.sources)
+    prob = bfs.CsrProblem()
+    b4._lib.check(prob.FromGraph(graph))
+    en = bfs.ENACTORS[args.strategy]()
+    t = tuning_for(args.strategy, graph, info["arch"])
+    if t:
+        en.set_tuning(**t)
+    for k in range(args.steps):
+        s = int(srcs[k % len(srcs)])
+        b4._lib.check(prob.Reset(en.GetFrontierType(), 1.3))
+        b4._lib.check(b4.lib().b40c_enact_search(en.handle, prob.handle, s, 0, b4.MODE_DEFAULT))
+        hip.sync()
+        log("probe: search %d from %d done" % (k, s))
+    prob.close()
+    en.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -287,12 +358,19 @@ def main():
     ap.add_argument("--cpu-sources", type=int, default=6, help="oracle BFS sources timed for cpu_baseline (0: skip)")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched 1,024-query workload")
     ap.add_argument("--no-served", action="store_true", help="skip the served Poisson workload")
+    ap.add_argument("--no-lattice", action="store_true", help="skip the 200^3 lattice (high-diameter) workload")
+    ap.add_argument("--probe", action="store_true",
+                    help="counter-collection run: a 1 GiB copy plus --steps searches, no JSON line")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print("bench.py measures one GPU only: every GPU box here has a single MI355X, and the "
               "multi-GPU partitioned BFS is out of scope (--gpus %d refused)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+
+    if args.probe:
+        probe(args)
+        return
 
     import back40computing_amd as b4
     from back40computing_amd import bfs
@@ -347,6 +425,13 @@ def main():
             per_strategy[names[s]] = round(r["gteps"], 3)
             log("strategy %s: %.3f GTEPS (%.3f ms/search)" % (names[s], r["gteps"], r["ms_per_search_enact"]))
 
+    lattice = None
+    if not args.no_lattice:
+        try:
+            lattice = bench_lattice(bfs, b4, hip, info)
+        except b4.B40CError as ex:
+            lattice = {"error": str(ex)}
+            log("lattice workload failed:", ex)
     batched = None if args.no_batched else bench_batched(b4, info)
     served = None
     if not args.no_served:
@@ -400,6 +485,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
         "cpu_baseline": cpu,
+        "lattice": lattice,
         "batched": batched,
         "served": served,
         "detail": {"gteps_enact_only": round(head["gteps_enact_only"], 3),

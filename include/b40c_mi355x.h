@@ -76,7 +76,6 @@ typedef struct b40c_tuning {
     int32_t do_beta;               /* direction-optimizing: back to top-down when n_f < n / beta   */
     int32_t persistent_wgs_per_cu; /* persistent kernels: resident workgroups per CU        */
     int32_t grid_gather_threshold; /* degree >= this: list split over all workgroups (grid tier) */
-    int32_t bottom_up_wave_threshold; /* bottom-up steps: longer lists are scanned by a whole wave */
 } b40c_tuning;
 
 /* ---- errors / device --------------------------------------------------- */

@@ -267,3 +267,26 @@ def test_rmat22_headline_sources(bfsmod, orc, rmat22, strategy):
         np.testing.assert_array_equal(depth, ref)
         assert orc.check_pred(off, cols, ref, pred) == -1
         assert st["search_depth"] == sd
+
+
+@pytest.fixture(scope="module")
+def lattice200(bfsmod):
+    """The high-diameter workload at full size: 200^3 6-neighbour lattice,
+    ids permuted from seed 0 (V = 8,000,000, E = 47,760,000)."""
+    g, perm = bfsmod.Graph.lattice3d(200, 0, want_perm=True)
+    off, cols = g.to_host()
+    return g, perm, off, cols
+
+
+@pytest.mark.parametrize("strategy", [b4.EXPAND_CONTRACT, b4.HYBRID, b4.DIRECTION_OPTIMIZING])
+def test_lattice200_closed_form(bfsmod, orc, lattice200, strategy):
+    """Full-size lattice: depth equals the Manhattan distance on the
+    unpermuted coordinates (about 600 levels from a corner)."""
+    g, perm, off, cols = lattice200
+    assert g.nodes == 8000000 and g.edges == 47760000
+    for s in (int(perm[0]), int(perm[200 ** 3 // 2 + 100])):  # a corner and a middle vertex
+        depth, pred, st = run(bfsmod, g, s, strategy, b4.MODE_DEFAULT)
+        ref = orc.lattice_depth(200, perm, s)
+        np.testing.assert_array_equal(depth, ref)
+        assert orc.check_pred(off, cols, ref, pred) == -1
+        assert st["search_depth"] == int(ref.max()) + 1
