@@ -59,6 +59,7 @@ class Tuning(ctypes.Structure):
         ("do_beta", ctypes.c_int32),
         ("persistent_wgs_per_cu", ctypes.c_int32),
         ("grid_gather_threshold", ctypes.c_int32),
+        ("bottom_up_grid_size", ctypes.c_int32),
     ]
 
     def as_dict(self):

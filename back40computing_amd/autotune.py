@@ -36,6 +36,7 @@ SPACE = {
     "grid_size": [256, 512, 1024, 2048, 4096],
     "levels_per_sync": [2, 4, 8, 12, 16],
     "saturation_quit": [64, 256, 512, 2048],
+    "bottom_up_grid_size": [1024, 2048, 4096],
 }
 
 # Knobs that matter per strategy.
@@ -48,7 +49,7 @@ KNOBS = {
                      "grid_size", "levels_per_sync"],
     _lib.HYBRID: ["saturation_quit", "grid_gather_threshold", "cta_gather_threshold", "wave_gather_threshold",
                   "grid_size"],
-    _lib.DIRECTION_OPTIMIZING: ["do_alpha", "do_beta", "grid_gather_threshold",
+    _lib.DIRECTION_OPTIMIZING: ["bottom_up_grid_size", "do_alpha", "do_beta", "grid_gather_threshold",
                                 "grid_size", "levels_per_sync", "wave_gather_threshold"],
 }
 

@@ -180,14 +180,15 @@ __device__ __forceinline__ void td_finish(const BfsArgs& a, int ph, TdSink& sink
 // culled and labelled at discovery (depth = depth_next), as in
 // expand_contract_atomic/cta.cuh:212-214, and compacted into the next vertex
 // frontier.
+// `nblocks` workgroups (the first ones of the launch) share the frontier.
 __device__ void td_phase(const BfsArgs& a, int ph, int32_t depth_next, uint32_t* next_bm, bool count_edges,
-                         LdsQueue<false>& q, GatherSmem& g, unsigned long long* red) {
+                         LdsQueue<false>& q, GatherSmem& g, unsigned long long* red, uint32_t nblocks) {
     const uint32_t n_in = phase_input_len(a, ph);
     phase_begin(a, ph, n_in);
     const int32_t* qin = a.qk[ph & 1];
     const GlobalQueue out = out_queue(a, ph);
     TdSink sink{a, q, out, next_bm, depth_next, count_edges, false, 0ull};
-    for (uint32_t base = blockIdx.x * BFS_THREADS; base < n_in; base += gridDim.x * BFS_THREADS) {
+    for (uint32_t base = blockIdx.x * BFS_THREADS; base < n_in; base += nblocks * BFS_THREADS) {
         const uint32_t i = base + threadIdx.x;
         int32_t u = -1, start = 0, deg = 0;
         if (i < n_in) {
@@ -480,11 +481,33 @@ __device__ __forceinline__ uint32_t do_direction(const BfsArgs& a, int ph, uint3
 
 enum SeedMode { SEED_VERTEX = 0, SEED_EDGE = 1, SEED_DO = 2 };
 
-// Resets the counters and enqueues the source (the iteration-0 prologue of
-// contract_expand_atomic/kernel.cuh:307-332 and expand_contract_atomic/
-// kernel.cuh:305-319).  Vertex modes label the source at once (depth 0,
-// pred -2); edge mode enqueues (src, -2) for labelling at contraction.
-__global__ void k_seed(BfsArgs a, int ph, int32_t src, int mode) {
+// Reset's label initialisation fused with the search prologue (the
+// reference's MemsetKernel x 2, csr_problem.cuh:723-745, then the
+// iteration-0 prologue of contract_expand_atomic/kernel.cuh:307-332 and
+// expand_contract_atomic/kernel.cuh:305-319) in one launch: depth and
+// predecessor := -1, visited mask and the first two frontier bitmaps := 0,
+// except the source, which vertex modes label at once (depth 0, pred -2,
+// visited, and its frontier bit for the direction-optimizing mode); edge
+// mode leaves it unlabelled and enqueues (src, -2) for contraction.  Every
+// value is written by exactly one thread, so no ordering is needed inside
+// the launch.  Workgroup 0 also resets the counters.
+__global__ __launch_bounds__(256) void k_reset_seed(BfsArgs a, int ph, int32_t src, int mode) {
+    const bool label_src = mode != SEED_EDGE;
+    const int32_t stride = gridDim.x * blockDim.x;
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < a.n; i += stride) {
+        const bool s = label_src && i == src;
+        a.depth[i] = s ? 0 : -1;
+        a.pred[i] = s ? -2 : -1;
+    }
+    const uint32_t sbit = label_src ? (1u << (src & 31)) : 0u;
+    const int32_t sw = src >> 5;
+    uint32_t* fcur = a.fbm[ph % 3];
+    uint32_t* fnext = a.fbm[(ph + 1) % 3];
+    for (int32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < a.words; w += stride) {
+        a.visited[w] = (w == sw) ? sbit : 0u;
+        fcur[w] = (mode == SEED_DO && w == sw) ? sbit : 0u;
+        fnext[w] = 0u;
+    }
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     DevCounters* c = a.ctr;
     for (int i = 0; i < 4; ++i) {
@@ -510,15 +533,10 @@ __global__ void k_seed(BfsArgs a, int ph, int32_t src, int mode) {
     c->corrupt = 0;
     *a.max_depth = 0;
     a.qk[ph & 1][0] = src;
-    if (mode == SEED_EDGE) {
+    if (mode == SEED_EDGE)
         a.qv[ph & 1][0] = -2;
-    } else {
-        a.visited[src >> 5] |= 1u << (src & 31);
-        a.depth[src] = 0;
-        a.pred[src] = -2;
+    else
         c->level_edges[ph & 3] = (unsigned long long)(a.offs[src + 1] - a.offs[src]);
-        if (mode == SEED_DO) a.fbm[ph % 3][src >> 5] |= 1u << (src & 31);
-    }
     c->qlen[ph & 3] = 1;
 }
 
@@ -575,7 +593,7 @@ __global__ __launch_bounds__(BFS_THREADS) void k_phase(BfsArgs a, int ph, int32_
     unsigned long long t0 = 0;
     stats_begin(t0, a);
     shared_init(s);
-    if (KIND == PK_TD) td_phase(a, ph, depth, nullptr, false, s.qv, s.g, s.red);
+    if (KIND == PK_TD) td_phase(a, ph, depth, nullptr, false, s.qv, s.g, s.red, gridDim.x);
     if (KIND == PK_CE) ce_phase(a, ph, depth, s);
     if (KIND == PK_TP_CONTRACT) tp_contract_phase(a, ph, depth, s);
     if (KIND == PK_TP_EXPAND) tp_expand_phase(a, ph, s);
@@ -607,10 +625,15 @@ __global__ __launch_bounds__(BFS_THREADS) void k_do_level(BfsArgs a, int ph, int
     uint32_t* clear = a.fbm[(ph + 2) % 3];
     for (int32_t w = blockIdx.x * BFS_THREADS + threadIdx.x; w < a.words; w += gridDim.x * BFS_THREADS) clear[w] = 0;
     if (dir == 0) {
-        if (threadIdx.x == 0) s.td.qv.count = 0;
-        gather_init(s.td.g);
-        __syncthreads();
-        td_phase(a, ph, ph + 1, a.fbm[(ph + 1) % 3], true, s.td.qv, s.td.g, s.td.red);
+        // Top-down steps use the first td_grid workgroups; the launch is
+        // sized for the bottom-up step, which needs every wave slot.
+        const uint32_t tdg = min(gridDim.x, (uint32_t)max(1, a.td_grid));
+        if (blockIdx.x < tdg) {
+            if (threadIdx.x == 0) s.td.qv.count = 0;
+            gather_init(s.td.g);
+            __syncthreads();
+            td_phase(a, ph, ph + 1, a.fbm[(ph + 1) % 3], true, s.td.qv, s.td.g, s.td.red, tdg);
+        }
     } else {
         phase_begin(a, ph, 0);
         if (threadIdx.x == 0) s.bu.q.count = 0;
@@ -734,7 +757,7 @@ __global__ __launch_bounds__(BFS_THREADS) void k_fused(BfsArgs a, int ph, int32_
         unsigned long long tb = 0;
         if (a.wg_stats && threadIdx.x == 0) tb = wg_clock();
         if (KIND == FK_TD) {
-            td_phase(a, ph, depth + 1, nullptr, false, s.qv, s.g, s.red);
+            td_phase(a, ph, depth + 1, nullptr, false, s.qv, s.g, s.red, gridDim.x);
             ph += 1;
         } else if (KIND == FK_CE) {
             ce_phase(a, ph, depth, s);

@@ -56,6 +56,7 @@ struct BfsArgs {
     int32_t symmetric;       // graph has every reverse edge (bottom-up allowed)
     int32_t do_alpha;
     int32_t do_beta;
+    int32_t td_grid;         // direction-optimizing: workgroups of a top-down step
     int32_t dev_base;        // 1: phase = ctr->ph_base + kernel argument (captured, replayed level loops)
     long long edges;
 };

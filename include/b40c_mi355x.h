@@ -76,6 +76,8 @@ typedef struct b40c_tuning {
     int32_t do_beta;               /* direction-optimizing: back to top-down when n_f < n / beta   */
     int32_t persistent_wgs_per_cu; /* persistent kernels: resident workgroups per CU        */
     int32_t grid_gather_threshold; /* degree >= this: list split over all workgroups (grid tier) */
+    int32_t bottom_up_grid_size;   /* direction-optimizing: workgroups per level launch, for its   */
+                                   /* bottom-up steps; top-down steps use grid_size of them       */
 } b40c_tuning;
 
 /* ---- errors / device --------------------------------------------------- */
