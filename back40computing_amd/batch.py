@@ -70,11 +70,16 @@ class MultiSourceBFS:
     def run(self):
         check(lib().b40c_msbfs_run(self._h))
 
-    def poll(self, max_ids=1 << 16):
-        ids = np.empty(max_ids, np.int32)
+    def poll(self, max_ids=1 << 12):
+        """Ids of finished queries not returned before (at most max_ids per call)."""
+        if getattr(self, "_poll_buf", None) is None or self._poll_buf.size < max_ids:
+            self._poll_buf = np.empty(max_ids, np.int32)
         c = ctypes.c_int32()
-        check(lib().b40c_msbfs_poll(self._h, ptr(ids), max_ids, ctypes.byref(c)))
-        return ids[:c.value].tolist()
+        check(lib().b40c_msbfs_poll(self._h, ptr(self._poll_buf), max_ids, ctypes.byref(c)))
+        out = self._poll_buf[:c.value].tolist()
+        if c.value == max_ids:
+            out += self.poll(max_ids)
+        return out
 
     def result(self, qid, graph, want_pred=True):
         n = self.mg.nodes(graph)

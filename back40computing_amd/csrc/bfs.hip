@@ -330,35 +330,69 @@ __device__ void tp_filter_phase(const BfsArgs& a, int ph, LevelShared& s) {
 // Bottom-up step of the direction-optimizing traversal (no counterpart in
 // the reference): every unvisited vertex looks for a neighbour in the current
 // frontier bitmap and stops at the first hit.  Only valid on symmetric
-// graphs.  The level kernel keeps a small LDS footprint so that many waves
-// are resident: the step is a chain of dependent loads (offsets -> column ->
-// bitmap word), so latency, not bandwidth, bounds it.
-// Each workgroup takes tiles of BFS_THREADS consecutive vertices:
+// graphs.  The step is a chain of dependent loads (visited word -> offsets
+// -> column -> bitmap word), so latency, not bandwidth, bounds it; every
+// wave therefore works on its own 64-vertex tiles with no workgroup
+// barriers, so a stalled wave never holds the others:
+//  * a tile whose 64 vertices are all visited is skipped after one load per
+//    lane (isolated vertices of symmetric graphs are pre-marked visited by
+//    the reset, so late levels skip almost every tile);
 //  * stage 1: each lane checks the first BU_FIRST neighbours of its vertex
 //    with independent loads;
 //  * stage 2: vertices still unresolved with neighbours left are scanned by
-//    BU_GROUP-lane groups, BU_GROUP neighbours per step, lowest lane wins
-//    (the first frontier neighbour in list order).
+//    BU_GROUP-lane groups of the wave, BU_GROUP neighbours per step, lowest
+//    lane wins (the first frontier neighbour in list order).
 // A wave owns the two bitmap words of its 64 vertices, so visited/next words
-// are written without atomics.  Discovered vertices are appended to the
-// vertex queue so that a top-down step can follow directly.
+// are written without atomics.  Discovered vertices go to the vertex queue
+// (one global atomic per wave tile) so that a top-down step can follow.
 constexpr int BU_FIRST = 4;
 constexpr int BU_GROUP = 8;
-constexpr int BU_GROUPS = BFS_THREADS / BU_GROUP;
+constexpr int BU_WAVE_GROUPS = kWave / BU_GROUP;
 
-struct BuShared {
-    LdsQueue<false> q;
-    int32_t pv[BFS_THREADS];  // unresolved vertex (tile-local index)
-    int32_t pe[BFS_THREADS];  // next edge to scan
-    int32_t pend_end[BFS_THREADS];
-    int32_t parent[BFS_THREADS];
-    unsigned long long red[BFS_WAVES];
-    uint32_t npend;
+struct BuWave {
+    int32_t start[kWave];  // pending entry: next edge to scan
+    int32_t end[kWave];
+    int32_t parent[kWave];
 };
 
+// Discovered vertices are buffered per wave and written to the global queue
+// with one atomic per workgroup at the end (or per wave when its buffer
+// fills), not one per tile: a single queue counter hit by every wave tile
+// serializes the whole step.
+constexpr int BU_QBUF = 512;
+
+struct BuShared {
+    BuWave w[BFS_WAVES];
+    int32_t qbuf[BFS_WAVES][BU_QBUF];
+    uint32_t qn[BFS_WAVES];
+    uint32_t qbase[BFS_WAVES];
+    unsigned long long red[BFS_WAVES];
+};
+
+// Copies `count` buffered vertices of this wave to the global queue at `base`.
+__device__ __forceinline__ void bu_copy_out(const int32_t* buf, uint32_t count, uint32_t base, int32_t* qout,
+                                            uint32_t qcap, uint32_t* overflow) {
+    for (uint32_t i = lane_id(); i < count; i += kWave) {
+        const uint32_t pos = base + i;
+        if (pos < qcap)
+            qout[pos] = buf[i];
+        else
+            st_relaxed(overflow, 1u);
+    }
+}
+
+// LDS written by some lanes of a wave and read by others after it: keep the
+// compiler from reordering across this point (a wave's LDS operations
+// complete in program order).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // LDS of a direction-optimizing level: the top-down step's queue and gather
-// staging or the bottom-up step's tile state (about 16.5 KB: eight
-// workgroups of 256 threads stay resident per CU).
+// staging or the bottom-up step's per-wave scratch (about 16.5 KB: eight
+// workgroups of 256 threads can stay resident per CU).
 struct DoTdShared {
     LdsQueue<false> qv;
     GatherSmem g;
@@ -371,25 +405,31 @@ union DoShared {
 
 __device__ __forceinline__ bool bm_test(const uint32_t* bm, int32_t u) { return (bm[u >> 5] >> (u & 31)) & 1u; }
 
-__device__ void bu_tiles(const BfsArgs& a, int ph, int32_t depth_next, const uint32_t* cur, uint32_t* next,
+__device__ void bu_waves(const BfsArgs& a, int ph, int32_t depth_next, const uint32_t* cur, uint32_t* next,
                          BuShared& s) {
-    const GlobalQueue out = out_queue(a, ph);
-    const int tid = threadIdx.x;
     const int lane = lane_id();
-    const int grp = tid / BU_GROUP, sub = tid % BU_GROUP;
+    const int wid = threadIdx.x / kWave;
+    BuWave& W = s.w[wid];
+    const int grp = lane / BU_GROUP, sub = lane % BU_GROUP;
+    const int gbase = lane & ~(BU_GROUP - 1);
+    int32_t* qout = a.qk[(ph + 1) & 1];
+    uint32_t* qlen = &a.ctr->qlen[(ph + 1) & 3];
+    const uint32_t qcap = a.qcap[(ph + 1) & 1];
+    int32_t* qb = s.qbuf[wid];
+    uint32_t wq = 0;  // vertices in this wave's buffer (wave-uniform)
     bool found_any = false;
     unsigned long long eacc = 0;
-    for (int64_t base = (int64_t)blockIdx.x * BFS_THREADS; base < a.n; base += (int64_t)gridDim.x * BFS_THREADS) {
-        if (tid == 0) s.npend = 0;
-        const int32_t v = (int32_t)(base + tid);
+    const int64_t nwaves = (int64_t)gridDim.x * BFS_WAVES;
+    for (int64_t vb = ((int64_t)blockIdx.x * BFS_WAVES + wid) * kWave; vb < a.n; vb += nwaves * kWave) {
+        const int32_t v = (int32_t)(vb + lane);
         const uint32_t vw = (v < a.n) ? a.visited[v >> 5] : 0xFFFFFFFFu;
         const bool active = (v < a.n) && !((vw >> (v & 31)) & 1u);
+        if (__ballot(active) == 0) continue;
         int32_t start = 0, deg = 0, parent = -1;
         if (active) {
             start = a.offs[v];
             deg = a.offs[v + 1] - start;
         }
-        __syncthreads();  // npend reset visible; previous tile's stage 2 done
         // Stage 1: the first BU_FIRST neighbours, loads issued together.
         if (deg > 0) {
             int32_t u[BU_FIRST];
@@ -398,34 +438,37 @@ __device__ void bu_tiles(const BfsArgs& a, int ph, int32_t depth_next, const uin
 #pragma unroll
             for (int k = BU_FIRST - 1; k >= 0; --k)
                 if (u[k] >= 0 && bm_test(cur, u[k])) parent = u[k];
-            if (parent < 0 && deg > BU_FIRST) {
-                const uint32_t slot = atomicAdd(&s.npend, 1u);
-                s.pv[slot] = tid;
-                s.pe[slot] = start + BU_FIRST;
-                s.pend_end[slot] = start + deg;
-            }
         }
-        s.parent[tid] = parent;
-        __syncthreads();
-        // Stage 2: groups of BU_GROUP lanes scan the unresolved lists.
-        const uint32_t np = s.npend;
-        for (uint32_t i = grp; i < np; i += BU_GROUPS) {
-            const int32_t e1 = s.pend_end[i];
-            int32_t hit_u = -1;
-            for (int32_t e0 = s.pe[i]; e0 < e1; e0 += BU_GROUP) {
-                const int32_t e = e0 + sub;
-                const int32_t u = (e < e1) ? a.cols[e] : -1;
-                const bool hit = u >= 0 && bm_test(cur, u);
-                const uint64_t hm = (__ballot(hit) >> (lane & ~(BU_GROUP - 1))) & ((1ull << BU_GROUP) - 1);
-                if (hm) {
-                    hit_u = __shfl(u, (lane & ~(BU_GROUP - 1)) + __ffsll((unsigned long long)hm) - 1, kWave);
-                    break;
+        // Stage 2: the wave's unresolved lists, one per BU_GROUP-lane group.
+        const bool pend = parent < 0 && deg > BU_FIRST;
+        const uint64_t pm = __ballot(pend);
+        if (pm) {
+            const int slot = lane_prefix(pm);
+            if (pend) {
+                W.start[slot] = start + BU_FIRST;
+                W.end[slot] = start + deg;
+            }
+            wave_lds_sync();
+            const int np = __popcll(pm);
+            for (int i = grp; i < np; i += BU_WAVE_GROUPS) {
+                const int32_t e1 = W.end[i];
+                int32_t hit_u = -1;
+                for (int32_t e0 = W.start[i]; e0 < e1; e0 += BU_GROUP) {
+                    const int32_t e = e0 + sub;
+                    const int32_t u = (e < e1) ? a.cols[e] : -1;
+                    const bool hit = u >= 0 && bm_test(cur, u);
+                    const uint64_t hm = (__ballot(hit) >> gbase) & ((1ull << BU_GROUP) - 1);
+                    if (hm) {
+                        hit_u = __shfl(u, gbase + __ffsll((unsigned long long)hm) - 1, kWave);
+                        break;
+                    }
                 }
+                if (sub == 0) W.parent[i] = hit_u;
             }
-            if (sub == 0 && hit_u >= 0) s.parent[s.pv[i]] = hit_u;
+            wave_lds_sync();
+            if (pend) parent = W.parent[slot];
+            wave_lds_sync();  // the next tile rewrites the scratch
         }
-        __syncthreads();
-        parent = s.parent[tid];
         const bool found = parent >= 0;
         if (found) {
             a.depth[v] = depth_next;
@@ -435,28 +478,43 @@ __device__ void bu_tiles(const BfsArgs& a, int ph, int32_t depth_next, const uin
         }
         const uint64_t fm = __ballot(found);
         if ((lane & 31) == 0) {
-            const int64_t w = ((base + tid) >> 5);
+            const int64_t w = v >> 5;
             if (w < a.words) {
                 const uint32_t bits = (lane == 0) ? (uint32_t)fm : (uint32_t)(fm >> 32);
                 next[w] = bits;
                 if (bits) a.visited[w] = vw | bits;
             }
         }
-        lq_append(s.q, out, found, v, 0);
-        lq_flush(s.q, out, LQ_CAP / 2);
+        if (fm) {
+            const uint32_t nf = (uint32_t)__popcll(fm);
+            if (wq + nf > (uint32_t)BU_QBUF) {  // buffer full: flush this wave's part now
+                uint32_t base = 0;
+                if (lane == 0) base = atomic_add_u32(qlen, wq);
+                base = (uint32_t)wave_bcast((int)base, 0);
+                bu_copy_out(qb, wq, base, qout, qcap, &a.ctr->overflow);
+                wq = 0;
+                wave_lds_sync();
+            }
+            if (found) qb[wq + (uint32_t)lane_prefix(fm)] = v;
+            wq += nf;
+        }
     }
-    lq_flush(s.q, out, 1);
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) eacc += __shfl_xor(eacc, d, kWave);
+    // One queue reservation for the whole workgroup's buffered vertices.
+    if (lane == 0) s.qn[wid] = wq;
     __syncthreads();
-    if (lane == 0) s.red[tid / kWave] = eacc;
-    __syncthreads();
-    if (tid == 0) {
-        unsigned long long t = 0;
-        for (int w = 0; w < BFS_WAVES; ++w) t += s.red[w];
-        if (t) atomic_add_u64(&a.ctr->level_edges[(ph + 1) & 3], t);
+    if (threadIdx.x == 0) {
+        uint32_t total = 0;
+        for (int w = 0; w < BFS_WAVES; ++w) {
+            s.qbase[w] = total;
+            total += s.qn[w];
+        }
+        const uint32_t base = total ? atomic_add_u32(qlen, total) : 0u;
+        for (int w = 0; w < BFS_WAVES; ++w) s.qbase[w] += base;
     }
-    if (__syncthreads_or(found_any) && tid == 0) atomic_max_u32(a.max_depth, (uint32_t)depth_next);
+    __syncthreads();
+    bu_copy_out(qb, wq, s.qbase[wid], qout, qcap, &a.ctr->overflow);
+    wg_add_u64(eacc, &a.ctr->level_edges[(ph + 1) & 3], s.red);
+    if (__syncthreads_or(found_any) && threadIdx.x == 0) atomic_max_u32(a.max_depth, (uint32_t)depth_next);
 }
 
 // Direction for level ph of the direction-optimizing traversal (Beamer,
@@ -504,7 +562,12 @@ __global__ __launch_bounds__(256) void k_reset_seed(BfsArgs a, int ph, int32_t s
     uint32_t* fcur = a.fbm[ph % 3];
     uint32_t* fnext = a.fbm[(ph + 1) % 3];
     for (int32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < a.words; w += stride) {
-        a.visited[w] = (w == sw) ? sbit : 0u;
+        // Isolated vertices of a symmetric graph can never be reached:
+        // marking them visited lets bottom-up steps skip them (their labels
+        // stay -1).  The source keeps only its own seeding.
+        uint32_t vis = a.iso ? a.iso[w] : 0u;
+        if (w == sw) vis = (vis & ~(1u << (src & 31))) | sbit;
+        a.visited[w] = vis;
         fcur[w] = (mode == SEED_DO && w == sw) ? sbit : 0u;
         fnext[w] = 0u;
     }
@@ -568,6 +631,17 @@ __global__ __launch_bounds__(256) void k_reset(int32_t* depth, int32_t* pred, ui
         f0[w] = 0;
         f1[w] = 0;
     }
+}
+
+// Bit v = (vertex v has no neighbours), one thread per 32-vertex word.
+__global__ __launch_bounds__(256) void k_isolated(const int32_t* offs, int32_t n, int32_t words, uint32_t* out) {
+    const int32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= words) return;
+    uint32_t bits = 0;
+    const int32_t v0 = w * 32;
+    for (int k = 0; k < 32 && v0 + k < n; ++k)
+        if (offs[v0 + k + 1] == offs[v0 + k]) bits |= 1u << k;
+    out[w] = bits;
 }
 
 // In a captured, replayed level loop (dev_base set) a kernel's phase
@@ -636,9 +710,7 @@ __global__ __launch_bounds__(BFS_THREADS) void k_do_level(BfsArgs a, int ph, int
         }
     } else {
         phase_begin(a, ph, 0);
-        if (threadIdx.x == 0) s.bu.q.count = 0;
-        __syncthreads();
-        bu_tiles(a, ph, ph + 1, a.fbm[ph % 3], a.fbm[(ph + 1) % 3], s.bu);
+        bu_waves(a, ph, ph + 1, a.fbm[ph % 3], a.fbm[(ph + 1) % 3], s.bu);
     }
     stats_end(t0, a, slot);
 }

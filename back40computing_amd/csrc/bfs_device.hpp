@@ -57,6 +57,7 @@ struct BfsArgs {
     int32_t do_alpha;
     int32_t do_beta;
     int32_t td_grid;         // direction-optimizing: workgroups of a top-down step
+    const uint32_t* iso;     // symmetric graphs: isolated-vertex bitmap, pre-marked visited (else null)
     int32_t dev_base;        // 1: phase = ctr->ph_base + kernel argument (captured, replayed level loops)
     long long edges;
 };

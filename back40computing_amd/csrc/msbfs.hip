@@ -712,7 +712,7 @@ struct b40c_msbfs {
     int32_t ph = 0;
     int32_t alpha = 14, beta = 24, grid = 1024;
     b40c::MsCounters* h_ctr = nullptr;
-    std::vector<unsigned long long> h_newbits;
+    unsigned long long* h_newbits = nullptr;  // pinned [G]
     int32_t qmax = 0;
 };
 
@@ -972,7 +972,7 @@ b40c_status b40c_msbfs_create(b40c_multigraph* mg, int64_t label_budget_bytes, i
     }
     m->arena.init(m->arena_elems);
     m->h_active.assign(m->G, 0ull);
-    m->h_newbits.assign((size_t)m->G, 0ull);
+    B40C_HIP(hipHostMalloc((void**)&m->h_newbits, (size_t)m->G * 8, hipHostMallocDefault));
     B40C_HIP(hipMemsetAsync(m->visited.ptr, 0, m->visited.bytes, m->stream));
     for (int i = 0; i < 3; ++i) B40C_HIP(hipMemsetAsync(m->front[i].ptr, 0, m->front[i].bytes, m->stream));
     B40C_HIP(hipMemsetAsync(m->newbits.ptr, 0, m->newbits.bytes, m->stream));
@@ -1032,7 +1032,7 @@ b40c_status b40c_msbfs_step(b40c_msbfs* m, int32_t* running, int32_t* waiting) {
                            a, m->ph);
         B40C_HIP(hipGetLastError());
         B40C_HIP(hipMemcpyAsync(m->h_ctr, m->counters.ptr, sizeof(MsCounters), hipMemcpyDeviceToHost, m->stream));
-        B40C_HIP(hipMemcpyAsync(m->h_newbits.data(), m->newbits.ptr, m->h_newbits.size() * 8, hipMemcpyDeviceToHost,
+        B40C_HIP(hipMemcpyAsync(m->h_newbits, m->newbits.ptr, (size_t)m->G * 8, hipMemcpyDeviceToHost,
                                 m->stream));
         B40C_HIP(hipStreamSynchronize(m->stream));
         if (m->h_ctr->overflow || m->h_ctr->corrupt)
@@ -1077,7 +1077,10 @@ b40c_status b40c_msbfs_result(b40c_msbfs* m, int32_t q, int32_t* depth, int32_t*
     if (depth)
         B40C_HIP(hipMemcpyAsync(depth, m->depth.as<int32_t>() + Q.lbase, vg * 4, hipMemcpyDeviceToHost, m->stream));
     if (pred) B40C_HIP(hipMemcpyAsync(pred, m->pred.as<int32_t>() + Q.lbase, vg * 4, hipMemcpyDeviceToHost, m->stream));
-    B40C_HIP(hipStreamSynchronize(m->stream));
+    // Without copies there is nothing to wait for: the slot's labels are no
+    // longer written (the query retired), and whoever reuses the memory next
+    // is ordered after everything already on the stream.
+    if (depth || pred) B40C_HIP(hipStreamSynchronize(m->stream));
     if (search_depth) *search_depth = Q.levels;
     m->arena.release(Q.lbase, vg);
     Q.state = 3;
@@ -1112,6 +1115,7 @@ void b40c_msbfs_destroy(b40c_msbfs* m) {
     m->seed_buf.release();
     if (m->h_seeds) (void)hipHostFree(m->h_seeds);
     if (m->h_ctr) (void)hipHostFree(m->h_ctr);
+    if (m->h_newbits) (void)hipHostFree(m->h_newbits);
     if (m->stream) (void)hipStreamDestroy(m->stream);
     delete m;
 }

@@ -116,31 +116,34 @@ class ContinuousServer:
 
 
 class PlainServer:
-    """FIFO server over the single-source enactor (one problem per graph)."""
+    """FIFO server over the single-source enactor: one resident problem and
+    one enactor per graph (created on first use), so each graph keeps its
+    tuned knobs and its captured level loop."""
 
     def __init__(self, graphs, strategy=_lib.DIRECTION_OPTIMIZING, tuning_for=None):
         self.graphs = graphs
         self.strategy = strategy
         self.tuning_for = tuning_for
         self.problems = {}
-        self.enactor = ENACTORS[strategy]()
+        self.enactors = {}
 
-    def _problem(self, g):
+    def _slot(self, g):
         p = self.problems.get(g)
         if p is None:
             p = CsrProblem()
             _lib.check(p.FromGraph(self.graphs[g]))
+            en = ENACTORS[self.strategy]()
+            t = self.tuning_for(g) if self.tuning_for else None
+            if t:
+                en.set_tuning(**t)
             self.problems[g] = p
-        return p
+            self.enactors[g] = en
+        return p, self.enactors[g]
 
     def run_one(self, g, src, want=False):
-        p = self._problem(g)
-        if self.tuning_for:
-            t = self.tuning_for(g)
-            if t:
-                self.enactor.set_tuning(**t)
-        _lib.check(p.Reset(self.enactor.GetFrontierType(), 1.3))
-        _lib.check(_lib.lib().b40c_enact_search(self.enactor.handle, p.handle, int(src), 0, _lib.MODE_DEFAULT))
+        p, en = self._slot(g)
+        _lib.check(p.Reset(en.GetFrontierType(), 1.3))
+        _lib.check(_lib.lib().b40c_enact_search(en.handle, p.handle, int(src), 0, _lib.MODE_DEFAULT))
         if want:
             return p.results()
         return None
@@ -163,4 +166,5 @@ class PlainServer:
     def close(self):
         for p in self.problems.values():
             p.close()
-        self.enactor.close()
+        for en in self.enactors.values():
+            en.close()

@@ -246,8 +246,8 @@ def bench_served(b4, info, duration_s=1.5):
         g = 1 + int((u - 0.9) / 0.1 * 512) % 512
         return (g, int(pool_src[g - 1][k % len(pool_src[g - 1])]))
 
-    tun = {0: autotune.lookup(info["arch"], b4.DIRECTION_OPTIMIZING, big.nodes, big.edges)}
-    plain = serve.PlainServer(graphs, b4.DIRECTION_OPTIMIZING, tuning_for=lambda g: tun.get(g))
+    plain = serve.PlainServer(graphs, b4.DIRECTION_OPTIMIZING, tuning_for=lambda g: autotune.lookup(
+        info["arch"], b4.DIRECTION_OPTIMIZING, graphs[g].nodes, graphs[g].edges))
     probe = [choose(serve._uniform(5, k), k) for k in range(40)]
     for g, s in probe[:8]:
         plain.run_one(g, s)
@@ -346,6 +346,23 @@ def probe(args):
     en.close()
 
 
+def run_child(which, timeout_s=420):
+    """Runs `bench.py --only <which>` in a child process (its progress goes to
+    our stderr) and returns its result dict, or an error record."""
+    import subprocess
+    cmd = [sys.executable, "-u", os.path.abspath(__file__), "--only", which]
+    try:
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, timeout=timeout_s)
+    except subprocess.TimeoutExpired:
+        log("%s workload: timed out after %d s" % (which, timeout_s))
+        return {"error": "timed out after %d s" % timeout_s}
+    for line in reversed(r.stdout.splitlines()):
+        if line.startswith("RESULT "):
+            return json.loads(line[len("RESULT "):])
+    log("%s workload: child exited with status %d and no result" % (which, r.returncode))
+    return {"error": "child exited with status %d" % r.returncode}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -359,6 +376,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched 1,024-query workload")
     ap.add_argument("--no-served", action="store_true", help="skip the served Poisson workload")
     ap.add_argument("--no-lattice", action="store_true", help="skip the 200^3 lattice (high-diameter) workload")
+    ap.add_argument("--only", choices=["batched", "served"], help=argparse.SUPPRESS)
     ap.add_argument("--probe", action="store_true",
                     help="counter-collection run: a 1 GiB copy plus --steps searches, no JSON line")
     args = ap.parse_args()
@@ -370,6 +388,13 @@ def main():
 
     if args.probe:
         probe(args)
+        return
+    if args.only:
+        import back40computing_amd as b4
+        from back40computing_amd import bfs
+        info = bfs.device_info()
+        res = bench_batched(b4, info) if args.only == "batched" else bench_served(b4, info)
+        print("RESULT " + json.dumps(res), flush=True)
         return
 
     import back40computing_amd as b4
@@ -432,14 +457,11 @@ def main():
         except b4.B40CError as ex:
             lattice = {"error": str(ex)}
             log("lattice workload failed:", ex)
-    batched = None if args.no_batched else bench_batched(b4, info)
-    served = None
-    if not args.no_served:
-        try:
-            served = bench_served(b4, info)
-        except (RuntimeError, b4.B40CError) as ex:
-            served = {"error": str(ex)}
-            log("served workload failed:", ex)
+    # The batched and served workloads run in child processes: they build
+    # their own graph pools, and whatever happens there cannot take the
+    # headline measurement down with it.
+    batched = None if args.no_batched else run_child("batched")
+    served = None if args.no_served else run_child("served")
 
     # ---- CPU baseline: the serial oracle BFS on a bounded sample ----------
     cpu = None
