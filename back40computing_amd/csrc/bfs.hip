@@ -109,8 +109,8 @@ __device__ __forceinline__ void wg_add_u64(unsigned long long v, unsigned long l
 // slots (high word) and HEAVY_CHUNK-edge chunk ranges (low word); each lane
 // records (vertex, first chunk) and points its chunks at its entry, so the
 // grid-tier kernel maps any chunk to its vertex with one load.
-__device__ __forceinline__ void defer_heavy(const BfsArgs& a, int ph, int32_t v, int32_t& deg) {
-    const bool heavy = deg >= a.grid_th;
+__device__ __forceinline__ void defer_heavy(const BfsArgs& a, int ph, int32_t v, int32_t& deg, int32_t grid_th) {
+    const bool heavy = deg >= grid_th;
     const uint64_t m = __ballot(heavy);
     if (m == 0) return;
     const int lane = lane_id();
@@ -180,9 +180,11 @@ __device__ __forceinline__ void td_finish(const BfsArgs& a, int ph, TdSink& sink
 // culled and labelled at discovery (depth = depth_next), as in
 // expand_contract_atomic/cta.cuh:212-214, and compacted into the next vertex
 // frontier.
-// `nblocks` workgroups (the first ones of the launch) share the frontier.
+// `nblocks` workgroups (the first ones of the launch) share the frontier;
+// lists of at least `grid_th` edges go to the grid tier (k_heavy).
 __device__ void td_phase(const BfsArgs& a, int ph, int32_t depth_next, uint32_t* next_bm, bool count_edges,
-                         LdsQueue<false>& q, GatherSmem& g, unsigned long long* red, uint32_t nblocks) {
+                         LdsQueue<false>& q, GatherSmem& g, unsigned long long* red, uint32_t nblocks,
+                         int32_t grid_th) {
     const uint32_t n_in = phase_input_len(a, ph);
     phase_begin(a, ph, n_in);
     const int32_t* qin = a.qk[ph & 1];
@@ -198,7 +200,7 @@ __device__ void td_phase(const BfsArgs& a, int ph, int32_t depth_next, uint32_t*
                 deg = a.offs[u + 1] - start;
             }
         }
-        defer_heavy(a, ph, u, deg);
+        defer_heavy(a, ph, u, deg, grid_th);
         tiered_gather(a.cols, u, start, deg, a.cta_th, a.wave_th, g, sink);
         lq_flush(q, out, LQ_CAP / 2);
     }
@@ -236,7 +238,7 @@ __device__ void ce_phase(const BfsArgs& a, int ph, int32_t depth_here, LevelShar
                 deg = a.offs[v + 1] - start;
             }
         }
-        defer_heavy(a, ph, v, deg);
+        defer_heavy(a, ph, v, deg, a.grid_th);
         tiered_gather(a.cols, v, start, deg, a.cta_th, a.wave_th, s.g, sink);
         lq_flush(s.qp, out, LQ_CAP / 2);
     }
@@ -295,7 +297,7 @@ __device__ void tp_expand_phase(const BfsArgs& a, int ph, LevelShared& s) {
                 deg = a.offs[u + 1] - start;
             }
         }
-        defer_heavy(a, ph, u, deg);
+        defer_heavy(a, ph, u, deg, a.grid_th);
         tiered_gather(a.cols, u, start, deg, a.cta_th, a.wave_th, s.g, sink);
         lq_flush(s.qp, out, LQ_CAP / 2);
     }
@@ -419,16 +421,37 @@ __device__ void bu_waves(const BfsArgs& a, int ph, int32_t depth_next, const uin
     uint32_t wq = 0;  // vertices in this wave's buffer (wave-uniform)
     bool found_any = false;
     unsigned long long eacc = 0;
-    const int64_t nwaves = (int64_t)gridDim.x * BFS_WAVES;
-    for (int64_t vb = ((int64_t)blockIdx.x * BFS_WAVES + wid) * kWave; vb < a.n; vb += nwaves * kWave) {
+    const int64_t step = (int64_t)gridDim.x * BFS_WAVES * kWave;
+    // The next tile's visited word and row offsets are loaded one iteration
+    // ahead (whether or not the vertex turns out to be visited), so the
+    // current tile's scan overlaps the next tile's first two round trips.
+    int64_t vb = ((int64_t)blockIdx.x * BFS_WAVES + wid) * kWave;
+    uint32_t nvw = 0xFFFFFFFFu;
+    int32_t noff0 = 0, noff1 = 0;
+    if (vb + lane < a.n) {
         const int32_t v = (int32_t)(vb + lane);
-        const uint32_t vw = (v < a.n) ? a.visited[v >> 5] : 0xFFFFFFFFu;
+        nvw = a.visited[v >> 5];
+        noff0 = a.offs[v];
+        noff1 = a.offs[v + 1];
+    }
+    for (; vb < a.n; vb += step) {
+        const int32_t v = (int32_t)(vb + lane);
+        const uint32_t vw = nvw;
+        const int32_t off0 = noff0, off1 = noff1;
+        if (vb + step + lane < a.n) {
+            const int32_t vn = (int32_t)(vb + step + lane);
+            nvw = a.visited[vn >> 5];
+            noff0 = a.offs[vn];
+            noff1 = a.offs[vn + 1];
+        } else {
+            nvw = 0xFFFFFFFFu;  // past the end: never active
+        }
         const bool active = (v < a.n) && !((vw >> (v & 31)) & 1u);
         if (__ballot(active) == 0) continue;
         int32_t start = 0, deg = 0, parent = -1;
         if (active) {
-            start = a.offs[v];
-            deg = a.offs[v + 1] - start;
+            start = off0;
+            deg = off1 - off0;
         }
         // Stage 1: the first BU_FIRST neighbours, loads issued together.
         if (deg > 0) {
@@ -667,7 +690,7 @@ __global__ __launch_bounds__(BFS_THREADS) void k_phase(BfsArgs a, int ph, int32_
     unsigned long long t0 = 0;
     stats_begin(t0, a);
     shared_init(s);
-    if (KIND == PK_TD) td_phase(a, ph, depth, nullptr, false, s.qv, s.g, s.red, gridDim.x);
+    if (KIND == PK_TD) td_phase(a, ph, depth, nullptr, false, s.qv, s.g, s.red, gridDim.x, a.grid_th);
     if (KIND == PK_CE) ce_phase(a, ph, depth, s);
     if (KIND == PK_TP_CONTRACT) tp_contract_phase(a, ph, depth, s);
     if (KIND == PK_TP_EXPAND) tp_expand_phase(a, ph, s);
@@ -679,7 +702,7 @@ __global__ __launch_bounds__(BFS_THREADS) void k_phase(BfsArgs a, int ph, int32_
 // the direction from device counters, clear the bitmap two levels ahead and
 // run a top-down or bottom-up step.  An empty frontier makes it a no-op, so
 // the host can launch levels ahead of the termination check.
-__global__ __launch_bounds__(BFS_THREADS) void k_do_level(BfsArgs a, int ph, int slot) {
+__global__ __launch_bounds__(BFS_THREADS) void k_do_level(BfsArgs a, int ph, int slot, int grid_tier) {
     __shared__ DoShared s;
     int32_t unused_depth = 0;
     resolve_phase(a, ph, unused_depth);
@@ -706,7 +729,8 @@ __global__ __launch_bounds__(BFS_THREADS) void k_do_level(BfsArgs a, int ph, int
             if (threadIdx.x == 0) s.td.qv.count = 0;
             gather_init(s.td.g);
             __syncthreads();
-            td_phase(a, ph, ph + 1, a.fbm[(ph + 1) % 3], true, s.td.qv, s.td.g, s.td.red, tdg);
+            td_phase(a, ph, ph + 1, a.fbm[(ph + 1) % 3], true, s.td.qv, s.td.g, s.td.red, tdg,
+                     grid_tier ? a.grid_th : INT_MAX);
         }
     } else {
         phase_begin(a, ph, 0);
@@ -829,7 +853,7 @@ __global__ __launch_bounds__(BFS_THREADS) void k_fused(BfsArgs a, int ph, int32_
         unsigned long long tb = 0;
         if (a.wg_stats && threadIdx.x == 0) tb = wg_clock();
         if (KIND == FK_TD) {
-            td_phase(a, ph, depth + 1, nullptr, false, s.qv, s.g, s.red, gridDim.x);
+            td_phase(a, ph, depth + 1, nullptr, false, s.qv, s.g, s.red, gridDim.x, a.grid_th);
             ph += 1;
         } else if (KIND == FK_CE) {
             ce_phase(a, ph, depth, s);
