@@ -346,6 +346,23 @@ def probe(args):
     en.close()
 
 
+def measured_traffic(workload, strategy):
+    """HBM bytes per search from the committed counter summary of the same
+    workload and strategy (two rocprofv3 --pmc passes, FETCH_SIZE and
+    WRITE_SIZE, each scaled by its reading for a copy of known size);
+    traffic is null when there is no summary for this configuration."""
+    path = os.path.join(ROOT, "profiles", "rmat22_direction_optimizing_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return {"traffic": None}
+    if t.get("workload") != workload or t.get("strategy") != strategy:
+        return {"traffic": None}
+    return {"traffic": t["traffic_bytes_per_search"], "traffic_unit": "bytes per search",
+            "traffic_source": "profiles/rmat22_direction_optimizing_traffic.json (commit %s)" % t.get("commit")}
+
+
 def run_child(which, timeout_s=420):
     """Runs `bench.py --only <which>` in a child process (its progress goes to
     our stderr) and returns its result dict, or an error record."""
@@ -487,6 +504,7 @@ def main():
         log("cpu baseline", cpu)
 
     achieved = head["achieved_gbs"]
+    out_workload = "rmat%d_ef16_a0.57_b0.19_c0.19_sym_dedup_single_source_bfs" % args.scale
     out = {
         "metric": "BFS GTEPS (RMAT scale-22) + batched-query p50/p99 vs plain, 1 MI355X",
         "value": round(head["gteps"], 3),
@@ -500,12 +518,13 @@ def main():
         "vs_baseline": None,
         "dtype": "int32",
         "data": "synthetic",
-        "config": {"workload": "rmat%d_ef16_a0.57_b0.19_c0.19_sym_dedup_single_source_bfs" % args.scale,
+        "config": {"workload": out_workload,
                    "vertices": n, "csr_edges": E, "sources": len(srcs), "strategy": names[args.strategy],
                    "tuning": head_tuning or "library defaults",
                    "step": "Reset + EnactSearch, depth + predecessor output"},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None},
+        "roofline": dict({"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": round(achieved / HBM_PEAK_GBS, 4)},
+                         **measured_traffic(out_workload, names[args.strategy])),
         "cpu_baseline": cpu,
         "lattice": lattice,
         "batched": batched,
