@@ -557,23 +557,31 @@ struct MsSeed {
     long long lbase;
 };
 
-// Registers newly admitted queries (slot table, active mask, label base,
-// start level, top-down mode, fresh direction statistics) and seeds them
-// into level ph: source bit in visited and the current mask array, source
-// labels, and the source appended to the current frontier queue if it was
-// not already on it.  The slot table and masks are only written here, in
-// k_ms_release and k_ms_direction, never copied from the host.
-__global__ void k_ms_seed(MsArgs a, int ph, const MsSeed* seeds, int count) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// Admits the queries of one level boundary, one row of workgroups per query
+// (blockIdx.y): the row initialises the query's label slot (depth and pred
+// -1, the source 0 and -2), and its first thread registers the query (slot
+// table, active mask, label base, start level, top-down mode, fresh
+// direction statistics) and seeds level ph: source bit in visited and the
+// current mask array, and the source appended to the current frontier queue
+// if it was not already on it.  The slot table and masks are only written
+// here, in k_ms_release and in k_ms_direction, never copied from the host.
+__global__ __launch_bounds__(256) void k_ms_admit(MsArgs a, int ph, const MsSeed* seeds, int count) {
+    const int i = blockIdx.y;
     if (i >= count) return;
     const MsSeed sd = seeds[i];
     const int32_t q = sd.q, v = sd.src;
     if ((uint32_t)v >= (uint32_t)a.n || (uint32_t)sd.bit >= 64u) {
-        st_relaxed(&a.ctr->corrupt, 1u);
+        if (blockIdx.x == 0 && threadIdx.x == 0) st_relaxed(&a.ctr->corrupt, 1u);
         return;
     }
-    const unsigned long long bit = 1ull << sd.bit;
     const int32_t g = a.gid[v];
+    const int32_t vb = a.vbase[g], vg = a.vbase[g + 1] - vb, ls = v - vb;
+    for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < vg; k += gridDim.x * blockDim.x) {
+        a.depth_out[sd.lbase + k] = (k == ls) ? 0 : -1;
+        a.pred_out[sd.lbase + k] = (k == ls) ? -2 : -1;
+    }
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const unsigned long long bit = 1ull << sd.bit;
     const size_t k = (size_t)g * 64 + sd.bit;
     a.bitq[k] = q;
     a.lbase[q] = sd.lbase;
@@ -583,9 +591,6 @@ __global__ void k_ms_seed(MsArgs a, int ph, const MsSeed* seeds, int count) {
     atomic_and_u64(&a.bu[g], ~bit);
     atomic_or_u64(&a.active[g], bit);
     atomic_or_u64(&a.visited[v], bit);
-    const long long idx = sd.lbase + (v - a.vbase[g]);
-    a.depth_out[idx] = 0;
-    a.pred_out[idx] = -2;
     const unsigned long long old = atomic_or_u64(&a.front[ph % 3][v], bit);
     if (old == 0ull) {
         const uint32_t pos = atomic_add_u32(&a.ctr->qlen[ph & 3], 1u);
@@ -596,22 +601,30 @@ __global__ void k_ms_seed(MsArgs a, int ph, const MsSeed* seeds, int count) {
     }
 }
 
-// Releases graph g's slot `bit`: slot table entry, active and bottom-up
-// bits, and the bit in visited over the graph's vertex range [v0, v1).
-__global__ void k_ms_release(MsArgs a, int32_t v0, int32_t v1, int32_t g, int32_t bit) {
-    const unsigned long long keep = ~(1ull << bit);
+// Slots of one graph retired at one level boundary.
+struct MsRelease {
+    int32_t g;
+    int32_t pad;
+    unsigned long long mask;
+};
+
+// Releases retired slots, one row of workgroups per graph (blockIdx.y):
+// slot table entries, active and bottom-up bits, and the bits in visited
+// over the graph's vertex range.
+__global__ __launch_bounds__(256) void k_ms_release(MsArgs a, const MsRelease* rel, int count) {
+    const int r = blockIdx.y;
+    if (r >= count) return;
+    const MsRelease R = rel[r];
+    if ((uint32_t)R.g >= (uint32_t)a.G) return;
+    const unsigned long long keep = ~R.mask;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
-        a.bitq[(size_t)g * 64 + bit] = -1;
-        atomic_and_u64(&a.active[g], keep);
-        atomic_and_u64(&a.bu[g], keep);
+        for (unsigned long long b = R.mask; b; b &= b - 1) a.bitq[(size_t)R.g * 64 + (__ffsll(b) - 1)] = -1;
+        atomic_and_u64(&a.active[R.g], keep);
+        atomic_and_u64(&a.bu[R.g], keep);
     }
+    const int32_t v0 = a.vbase[R.g], v1 = a.vbase[R.g + 1];
     for (int32_t v = v0 + blockIdx.x * blockDim.x + threadIdx.x; v < v1; v += gridDim.x * blockDim.x)
         a.visited[v] &= keep;
-}
-
-__global__ void k_fill_i32(int32_t* p, int32_t value, long long n) {
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
-        p[i] = value;
 }
 
 // Packs several CSRs into one: vertex ids shifted by the graph's base.
@@ -699,8 +712,10 @@ struct b40c_msbfs {
     int64_t arena_elems = 0;  // per label array (depth and pred each)
     b40c::DeviceBuffer visited, front[3], active, bu, newbits, cnt_n, cnt_m, nf_prev, m_seen, bitq, lbase, qstart, depth,
         pred, queue[2], counters;
-    b40c::DeviceBuffer seed_buf, heavy, chunk_map, fbm;
+    b40c::DeviceBuffer seed_buf, rel_buf, heavy, chunk_map, fbm;
     b40c::MsSeed* h_seeds = nullptr;  // pinned staging
+    b40c::MsRelease* h_rel = nullptr;  // pinned staging
+    int h_rel_cap = 0;
     int h_seed_cap = 0;
     uint32_t qcap = 0, heavy_cap = 0, chunk_cap = 0;
     b40c::ArenaAlloc arena;
@@ -766,6 +781,7 @@ unsigned grid_for_n(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<
 b40c_status ms_admit(b40c_msbfs* m) {
     std::vector<MsSeed> seeds;
     std::deque<int32_t> still;
+    int64_t max_vg = 1;
     while (!m->waiting.empty()) {
         const int32_t q = m->waiting.front();
         m->waiting.pop_front();
@@ -788,11 +804,7 @@ b40c_status ms_admit(b40c_msbfs* m) {
         Q.qstart = m->ph;
         Q.state = 1;
         m->h_active[g] |= 1ull << b;
-        // Label slot := -1 for depth and pred.
-        hipLaunchKernelGGL(k_fill_i32, dim3(grid_for_n(vg)), dim3(256), 0, m->stream, m->depth.as<int32_t>() + lb, -1,
-                           (long long)vg);
-        hipLaunchKernelGGL(k_fill_i32, dim3(grid_for_n(vg)), dim3(256), 0, m->stream, m->pred.as<int32_t>() + lb, -1,
-                           (long long)vg);
+        max_vg = std::max(max_vg, vg);
         seeds.push_back(MsSeed{q, m->mg->vbase[g] + Q.src, b, m->ph, (long long)lb});
         m->running.push_back(q);
     }
@@ -811,17 +823,23 @@ b40c_status ms_admit(b40c_msbfs* m) {
     std::memcpy(m->h_seeds, seeds.data(), sizeof(MsSeed) * cnt);
     B40C_TRY(m->seed_buf.reserve(sizeof(MsSeed) * (size_t)m->h_seed_cap));
     B40C_HIP(hipMemcpyAsync(m->seed_buf.ptr, m->h_seeds, sizeof(MsSeed) * cnt, hipMemcpyHostToDevice, m->stream));
+    // One launch for every admitted query: a row of workgroups per query
+    // (at most 65535 rows per launch), each row filling its label slot.
     const MsArgs a = ms_args(m);
-    hipLaunchKernelGGL(k_ms_seed, dim3((cnt + 255) / 256), dim3(256), 0, m->stream, a, m->ph,
-                       m->seed_buf.as<MsSeed>(), cnt);
-    B40C_HIP(hipGetLastError());
+    const unsigned gx = (unsigned)std::min<int64_t>((max_vg + 255) / 256, 1024);
+    for (int r0 = 0; r0 < cnt; r0 += 65535) {
+        const int rows = std::min(cnt - r0, 65535);
+        hipLaunchKernelGGL(k_ms_admit, dim3(gx, (unsigned)rows), dim3(256), 0, m->stream, a, m->ph,
+                           m->seed_buf.as<MsSeed>() + r0, rows);
+        B40C_HIP(hipGetLastError());
+    }
     return B40C_OK;
 }
 
 // Retires queries whose slot found nothing new in the level just run.
 b40c_status ms_retire(b40c_msbfs* m) {
     std::vector<int32_t> keep;
-    const MsArgs a = ms_args(m);
+    std::map<int32_t, unsigned long long> masks;  // graph -> retired slots
     for (int32_t q : m->running) {
         MsQuery& Q = m->queries[q];
         const unsigned long long bit = 1ull << Q.bit;
@@ -833,12 +851,37 @@ b40c_status ms_retire(b40c_msbfs* m) {
         Q.state = 2;
         m->completed.push_back(q);
         m->h_active[Q.graph] &= ~bit;
-        const int32_t v0 = m->mg->vbase[Q.graph], v1 = m->mg->vbase[Q.graph + 1];
-        hipLaunchKernelGGL(k_ms_release, dim3(grid_for_n(v1 - v0)), dim3(256), 0, m->stream, a, v0, v1, Q.graph,
-                           Q.bit);
-        B40C_HIP(hipGetLastError());
+        masks[Q.graph] |= bit;
     }
     m->running.swap(keep);
+    if (masks.empty()) return B40C_OK;
+    // One launch for all graphs with retired slots, a row of workgroups per
+    // graph.  Pinned staging: the step synchronized the stream, so the
+    // previous level's release has completed.
+    const int cnt = (int)masks.size();
+    if (m->h_rel_cap < cnt) {
+        if (m->h_rel) (void)hipHostFree(m->h_rel);
+        m->h_rel = nullptr;
+        m->h_rel_cap = 0;
+        B40C_HIP(hipHostMalloc((void**)&m->h_rel, sizeof(MsRelease) * (size_t)std::max(cnt, 256), 0));
+        m->h_rel_cap = std::max(cnt, 256);
+    }
+    int i = 0;
+    int64_t max_range = 1;
+    for (const auto& gm : masks) {
+        m->h_rel[i++] = MsRelease{gm.first, 0, gm.second};
+        max_range = std::max<int64_t>(max_range, m->mg->vbase[gm.first + 1] - m->mg->vbase[gm.first]);
+    }
+    B40C_TRY(m->rel_buf.reserve(sizeof(MsRelease) * (size_t)m->h_rel_cap));
+    B40C_HIP(hipMemcpyAsync(m->rel_buf.ptr, m->h_rel, sizeof(MsRelease) * cnt, hipMemcpyHostToDevice, m->stream));
+    const MsArgs a = ms_args(m);
+    const unsigned gx = (unsigned)std::min<int64_t>((max_range + 255) / 256, 1024);
+    for (int r0 = 0; r0 < cnt; r0 += 65535) {
+        const int rows = std::min(cnt - r0, 65535);
+        hipLaunchKernelGGL(k_ms_release, dim3(gx, (unsigned)rows), dim3(256), 0, m->stream, a,
+                           m->rel_buf.as<MsRelease>() + r0, rows);
+        B40C_HIP(hipGetLastError());
+    }
     return B40C_OK;
 }
 
@@ -1113,7 +1156,9 @@ void b40c_msbfs_destroy(b40c_msbfs* m) {
     m->fbm.release();
     m->counters.release();
     m->seed_buf.release();
+    m->rel_buf.release();
     if (m->h_seeds) (void)hipHostFree(m->h_seeds);
+    if (m->h_rel) (void)hipHostFree(m->h_rel);
     if (m->h_ctr) (void)hipHostFree(m->h_ctr);
     if (m->h_newbits) (void)hipHostFree(m->h_newbits);
     if (m->stream) (void)hipStreamDestroy(m->stream);
