@@ -56,6 +56,10 @@ class MultiSourceBFS:
     def set_tuning(self, alpha=0, beta=0, grid=0):
         check(lib().b40c_msbfs_set_tuning(self._h, alpha, beta, grid))
 
+    def set_admission(self, cohort_gap):
+        """Levels between admissions while queries run (1: every boundary)."""
+        check(lib().b40c_msbfs_set_admission(self._h, int(cohort_gap)))
+
     def submit(self, graph, src):
         q = ctypes.c_int32()
         check(lib().b40c_msbfs_submit(self._h, int(graph), int(src), ctypes.byref(q)))

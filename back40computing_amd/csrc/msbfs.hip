@@ -726,6 +726,8 @@ struct b40c_msbfs {
     std::vector<int32_t> completed;            // finished, not yet reported
     int32_t ph = 0;
     int32_t alpha = 14, beta = 24, grid = 1024;
+    int32_t cohort_gap = 6;        // levels between admissions while queries run
+    int32_t last_admit_ph = -1000;
     b40c::MsCounters* h_ctr = nullptr;
     unsigned long long* h_newbits = nullptr;  // pinned [G]
     int32_t qmax = 0;
@@ -779,6 +781,11 @@ unsigned grid_for_n(int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<
 
 // Admits waiting queries while a slot of their graph and label memory are free.
 b40c_status ms_admit(b40c_msbfs* m) {
+    // Cohorts: queries that start together share their expensive first
+    // levels (top-down hub expansion, the first bottom-up sweep); starting a
+    // few queries at every boundary would pay those levels again each time.
+    if (m->waiting.empty()) return B40C_OK;
+    if (!m->running.empty() && m->ph - m->last_admit_ph < m->cohort_gap) return B40C_OK;
     std::vector<MsSeed> seeds;
     std::deque<int32_t> still;
     int64_t max_vg = 1;
@@ -810,6 +817,7 @@ b40c_status ms_admit(b40c_msbfs* m) {
     }
     m->waiting.swap(still);
     if (seeds.empty()) return B40C_OK;
+    m->last_admit_ph = m->ph;
     const int cnt = (int)seeds.size();
     // Pinned staging: the previous step synchronized the stream, so the
     // buffer is free to overwrite.
@@ -1038,6 +1046,12 @@ b40c_status b40c_msbfs_set_tuning(b40c_msbfs* m, int32_t alpha, int32_t beta, in
     if (alpha > 0) m->alpha = alpha;
     if (beta > 0) m->beta = beta;
     if (grid > 0) m->grid = grid;
+    return B40C_OK;
+}
+
+b40c_status b40c_msbfs_set_admission(b40c_msbfs* m, int32_t cohort_gap) {
+    if (!m || cohort_gap < 1) return fail(B40C_ERR_INVALID_ARG, "b40c_msbfs_set_admission: bad arguments");
+    m->cohort_gap = cohort_gap;
     return B40C_OK;
 }
 

@@ -266,7 +266,7 @@ def bench_served(b4, info, duration_s=1.5):
                           "achieved_qps": round(len(trace) / wall, 1), "p50_ms": round(pct(lat, 50) * 1e3, 3),
                           "p99_ms": round(pct(lat, 99) * 1e3, 3)})
     cont = serve.ContinuousServer(mg, label_budget_bytes=6 << 30, log=log)
-    for mult in (2, 5, 10, 20):
+    for mult in (2, 3, 4, 5, 10):
         rate = mult * rate_plain
         trace = serve.poisson_trace(rate, duration_s, choose, seed=21 + mult)
         log("served: continuous trace of %d queries at %.0f qps" % (len(trace), rate))

@@ -188,6 +188,11 @@ b40c_status b40c_msbfs_create(b40c_multigraph* mg, int64_t label_budget_bytes, i
                               b40c_msbfs** out);
 /* Direction switch parameters and workgroups per level launch (0 keeps). */
 b40c_status b40c_msbfs_set_tuning(b40c_msbfs* m, int32_t alpha, int32_t beta, int32_t grid);
+/* Admission policy: while queries are running, waiting queries join only at
+ * level boundaries at least `cohort_gap` levels after the last admission, so
+ * queries arrive in cohorts that share their expensive first levels (1 =
+ * admit at every boundary; default 6).  An idle engine admits at once. */
+b40c_status b40c_msbfs_set_admission(b40c_msbfs* m, int32_t cohort_gap);
 /* Queues a query (source `src` local to graph `graph`); returns its id.
  * B40C_ERR_ADMISSION if its labels can never fit the budget. */
 b40c_status b40c_msbfs_submit(b40c_msbfs* m, int32_t graph, int32_t src, int32_t* query_id);

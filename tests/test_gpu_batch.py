@@ -128,3 +128,34 @@ def test_admission_errors(mods, orc, pool):
     with pytest.raises(B40CError) as ei:
         batch.MultiSourceBFS(dm, label_budget_bytes=1 << 20)
     assert ei.value.status == _lib.ERR_UNSUPPORTED
+
+
+def test_admission_policy_does_not_change_labels(mods, orc, pool):
+    """Cohort admission (the default gap) and admission at every level
+    boundary give the same labels for queries that arrive over time."""
+    bfs, batch = mods
+    mg, host = pool
+    arrivals = [[(g, s) for s in sources(orc, host, g, 3, 40 + g)] for g in range(mg.ngraphs)]
+    results = []
+    for gap in (1, 6):
+        eng = batch.MultiSourceBFS(mg, label_budget_bytes=1 << 28)
+        eng.set_admission(gap)
+        ids, out = {}, {}
+        for wave in arrivals:  # one graph's queries per level boundary
+            for g, s in wave:
+                ids[eng.submit(g, s)] = (g, s)
+            eng.step()
+            for q in eng.poll():
+                out[ids[q]] = eng.result(q, ids[q][0])[0]
+        eng.run()
+        for q in eng.poll():
+            out[ids[q]] = eng.result(q, ids[q][0])[0]
+        assert len(out) == len(ids)
+        results.append(out)
+    for key, depth in results[0].items():
+        np.testing.assert_array_equal(depth, results[1][key])
+        g, s = key
+        ref, _, _ = orc.bfs(host[g][0], host[g][1], int(s), with_pred=False)
+        np.testing.assert_array_equal(depth, ref)
+    with pytest.raises(Exception):
+        batch.MultiSourceBFS(mg, label_budget_bytes=1 << 20).set_admission(0)
