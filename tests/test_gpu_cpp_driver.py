@@ -32,3 +32,16 @@ def test_cpp_driver(case):
     assert r.returncode == 0, r.stdout + r.stderr
     assert "failures=0" in r.stdout
     assert r.stdout.count("CORRECT") >= 3
+
+
+def test_cpp_sort_driver():
+    """The radix sort through the C++ facade (include/b40c/radix_sort.hpp):
+    64-bit keys (all bits and a bit range) and stable 32-bit pairs against
+    std::sort / std::stable_sort (test/radix_sort/test_sort.cu:185-261)."""
+    drv = os.path.join(os.path.dirname(DRIVER), "test_sort")
+    if not os.path.exists(drv):
+        pytest.fail("C++ sort driver not built (run __graft_entry__.build())")
+    r = subprocess.run([drv, "3000000", "11"], capture_output=True, text=True, timeout=120)
+    print(r.stdout[-2000:])
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "all CORRECT" in r.stdout

@@ -79,5 +79,6 @@ def test_cpp_driver_built_and_usage():
     import subprocess
     drv = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "test_bfs")
     assert os.path.exists(drv), "C++ driver not built"
+    assert os.path.exists(os.path.join(os.path.dirname(drv), "test_sort")), "C++ sort driver not built"
     r = subprocess.run([drv], capture_output=True, text=True, timeout=60)
     assert r.returncode == 2 and "usage:" in r.stderr
