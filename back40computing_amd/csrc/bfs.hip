@@ -677,8 +677,22 @@ __device__ __forceinline__ void resolve_phase(const BfsArgs& a, int& ph, int32_t
     }
 }
 
+// Last node of a captured level loop: advances the phase base and publishes
+// the counters to the pinned host mirror, so the host's per-replay check
+// needs no separate device-to-host copy.
 __global__ void k_advance(BfsArgs a, int steps) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) a.ctr->ph_base += (uint32_t)steps;
+    constexpr int kWords = (int)(sizeof(DevCounters) / 4);
+    constexpr int kPhWord = (int)(offsetof(DevCounters, ph_base) / 4);
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(a.ctr);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(a.hctr);
+    for (int i = threadIdx.x; i < kWords; i += blockDim.x) {
+        uint32_t v = src[i];
+        if (i == kPhWord) {
+            v += (uint32_t)steps;
+            reinterpret_cast<uint32_t*>(a.ctr)[i] = v;
+        }
+        dst[i] = v;
+    }
 }
 
 enum PhaseKind { PK_TD = 0, PK_CE = 2, PK_TP_CONTRACT = 3, PK_TP_EXPAND = 4, PK_TP_FILTER = 5 };

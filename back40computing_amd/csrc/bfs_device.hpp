@@ -46,6 +46,7 @@ struct BfsArgs {
     int32_t* qv[2];          // frontier queue values (predecessors), edge frontiers only
     uint32_t qcap[2];        // capacities of qk[i] (elements)
     DevCounters* ctr;
+    DevCounters* hctr;       // device address of the pinned host mirror of *ctr
     uint32_t* max_depth;     // deepest depth label written
     unsigned long long* wg_stats;  // instrumented: [grid][2] (busy clocks, lifetime clocks)
     int32_t n;
