@@ -48,9 +48,9 @@ struct alignas(256) MsCounters {
     uint32_t any_bu[2];  // some slot is in bottom-up mode (by level parity)
     uint32_t overflow;
     uint32_t corrupt;
-    uint32_t ph_base;
-    uint32_t pad[7];
+    uint32_t pad[8];
     unsigned long long heavy_alloc[2];  // (entries << 32 | chunks) deferred to the grid tier, by parity
+    unsigned long long bu_tiles[2];     // (graphs << 32 | tiles) listed for the pull, by parity
 };
 
 struct MsArgs {
@@ -73,8 +73,11 @@ struct MsArgs {
     int32_t* qstart;                   // [Q] level the query joined
     int32_t* depth_out;
     int32_t* pred_out;
-    int32_t* queue[2];
-    uint32_t* fbm;                     // [ceil(n/32)] vertices carrying any bit in the current masks
+    int32_t* queue[3];                 // frontier queues, rotating by level % 3 (front[] pairs with queue[])
+    uint32_t* fbm[2];                  // [ceil(n/32)] by level parity: vertices carrying any bit in front[]
+    int2* bu_list;                     // [2][G] by level parity: (graph, first tile) of graphs with bottom-up slots
+    MsCounters* hctr;                  // device address of the pinned host mirror (overflow, corrupt)
+    unsigned long long* hnewbits;      // device address of the pinned host copy of newbits
     int2* heavy;                       // grid-tier entries (vertex, first chunk)
     int32_t* chunk_map;                // chunk -> entry
     uint32_t qcap;
@@ -88,7 +91,6 @@ struct MsArgs {
     int32_t cta_th;
     int32_t wave_th;
     int32_t grid_th;
-    int32_t dev_base;
 };
 
 __device__ __forceinline__ unsigned long long atomic_or_u64(unsigned long long* p, unsigned long long v) {
@@ -116,6 +118,14 @@ __device__ __forceinline__ unsigned long long wave_transpose64(unsigned long lon
     return x;
 }
 
+// A per-level pass over the vertices of a frontier queue replaces a pass over
+// all n vertices while the queue is shorter than n / MS_SPARSE_DIV.
+constexpr int MS_SPARSE_DIV = 16;
+// The pull walks the tiles of at most this many graphs through an LDS table
+// (more: every tile of the pack).
+constexpr int MS_TILE_TABLE = 512;
+constexpr int MS_TILE = BFS_THREADS;  // vertices per pull tile
+
 struct MsShared {
     LdsQueue<false> q;
     GatherSmem g;
@@ -140,7 +150,7 @@ __device__ __forceinline__ void ms_record(const MsArgs& a, int ph, int32_t v, in
 
 __device__ __forceinline__ GlobalQueue ms_out_queue(const MsArgs& a, int ph) {
     GlobalQueue out;
-    out.keys = a.queue[(ph + 1) & 1];
+    out.keys = a.queue[(ph + 1) % 3];
     out.vals = nullptr;
     out.len = &a.ctr->qlen[(ph + 1) & 3];
     out.cap = a.qcap;
@@ -207,16 +217,29 @@ __device__ __forceinline__ void ms_defer_heavy(const MsArgs& a, int ph, int32_t 
 // enqueues) to the next frontier queue.
 __global__ __launch_bounds__(BFS_THREADS) void k_ms_push(MsArgs a, int ph) {
     __shared__ MsShared s;
-    if (a.dev_base) ph += (int)ld_relaxed(&a.ctr->ph_base);
     uint32_t n_f = ld_relaxed(&a.ctr->qlen[ph & 3]);
     if (n_f > a.qcap) n_f = 0;
-    // Clear the mask array two levels ahead, its queue length and the next
-    // level's grid-tier allocation.
+    // Clear the mask array of level ph - 1 (it becomes level ph + 2's next
+    // array).  Its nonzero entries are exactly the vertices on level ph - 1's
+    // queue (every first OR enqueues), so a short queue is cleared entry by
+    // entry and a long one by a full pass.  Also the queue length two levels
+    // ahead, the next level's grid-tier allocation and its pull tile list.
     unsigned long long* clear = a.front[(ph + 2) % 3];
-    for (int32_t v = blockIdx.x * BFS_THREADS + threadIdx.x; v < a.n; v += gridDim.x * BFS_THREADS) clear[v] = 0ull;
+    const uint32_t n_old = ld_relaxed(&a.ctr->qlen[(ph + 3) & 3]);
+    const int32_t gtid = blockIdx.x * BFS_THREADS + threadIdx.x, gstride = gridDim.x * BFS_THREADS;
+    if (n_old <= a.qcap && (int64_t)n_old * MS_SPARSE_DIV < (int64_t)a.n) {
+        const int32_t* qold = a.queue[(ph + 2) % 3];
+        for (int32_t i = gtid; i < (int32_t)n_old; i += gstride) {
+            const int32_t v = qold[i];
+            if ((uint32_t)v < (uint32_t)a.n) clear[v] = 0ull;
+        }
+    } else {
+        for (int32_t v = gtid; v < a.n; v += gstride) clear[v] = 0ull;
+    }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         st_relaxed(&a.ctr->qlen[(ph + 2) & 3], 0u);
         __hip_atomic_store(&a.ctr->heavy_alloc[(ph + 1) & 1], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&a.ctr->bu_tiles[(ph + 1) & 1], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     if (n_f == 0) return;  // no vertex carries a bit: nothing can be found
     if (threadIdx.x == 0) s.q.count = 0;
@@ -226,7 +249,7 @@ __global__ __launch_bounds__(BFS_THREADS) void k_ms_push(MsArgs a, int ph) {
     const unsigned long long* cur = a.front[ph % 3];
 
     // Top-down push over the frontier queue, for the bits in top-down mode.
-    const int32_t* qin = a.queue[ph & 1];
+    const int32_t* qin = a.queue[ph % 3];
     for (uint32_t base = blockIdx.x * BFS_THREADS; base < n_f; base += gridDim.x * BFS_THREADS) {
         const uint32_t i = base + threadIdx.x;
         int32_t u = -1, start = 0, deg = 0;
@@ -267,6 +290,7 @@ constexpr int MS_GROUPS = BFS_THREADS / MS_GROUP;
 
 struct MsPullShared {
     LdsQueue<false> q;
+    int2 tab[MS_TILE_TABLE];  // (graph, first tile) of the graphs with bottom-up slots
     int32_t pv[BFS_THREADS];  // unresolved vertex (tile-local index)
     int32_t pe[BFS_THREADS];  // next edge to scan
     int32_t pend_end[BFS_THREADS];
@@ -297,26 +321,56 @@ __device__ __forceinline__ unsigned long long group_excl_or(unsigned long long x
 
 __global__ __launch_bounds__(BFS_THREADS) void k_ms_pull(MsArgs a, int ph) {
     __shared__ MsPullShared s;
-    if (a.dev_base) ph += (int)ld_relaxed(&a.ctr->ph_base);
     uint32_t n_f = ld_relaxed(&a.ctr->qlen[ph & 3]);
     if (n_f > a.qcap) n_f = 0;
     if (n_f == 0 || !ld_relaxed(&a.ctr->any_bu[ph & 1])) return;
+    // Tiles: with the level's graph table (k_ms_direction lists every graph
+    // with bottom-up slots, in tile order), tile t lies in graph tab[e].x for
+    // the last e with tab[e].y <= t; else tile t is vertices [t, t+1) * MS_TILE
+    // of the pack.
+    const unsigned long long tl =
+        __hip_atomic_load(&a.ctr->bu_tiles[ph & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int32_t ne = (int32_t)(tl >> 32);
+    const bool table = ne <= MS_TILE_TABLE && ne <= a.G;
+    for (int i = threadIdx.x; table && i < ne; i += BFS_THREADS) s.tab[i] = a.bu_list[(size_t)(ph & 1) * a.G + i];
+    const uint32_t ntiles = table ? (uint32_t)tl : (uint32_t)(((int64_t)a.n + MS_TILE - 1) / MS_TILE);
     if (threadIdx.x == 0) s.q.count = 0;
     __syncthreads();
     const GlobalQueue out = ms_out_queue(a, ph);
     const unsigned long long* cur = a.front[ph % 3];
     unsigned long long* next = a.front[(ph + 1) % 3];
-    const uint32_t* fbm = a.fbm;
+    const uint32_t* fbm = a.fbm[ph & 1];
     const int tid = threadIdx.x;
     const int lane = lane_id();
     const int grp = tid / MS_GROUP, sub = tid % MS_GROUP;
-    for (int64_t base = (int64_t)blockIdx.x * BFS_THREADS; base < a.n; base += (int64_t)gridDim.x * BFS_THREADS) {
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        int32_t base, end, tg = -1;
+        if (table) {
+            int lo = 0, hi = ne - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if ((uint32_t)s.tab[mid].y <= t) lo = mid;
+                else hi = mid - 1;
+            }
+            tg = s.tab[lo].x;
+            if ((uint32_t)tg < (uint32_t)a.G) {
+                base = a.vbase[tg] + (int32_t)(t - (uint32_t)s.tab[lo].y) * MS_TILE;
+                end = min(base + MS_TILE, a.vbase[tg + 1]);
+            } else {
+                if (tid == 0) st_relaxed(&a.ctr->corrupt, 1u);
+                tg = -1;
+                base = end = 0;
+            }
+        } else {
+            base = (int32_t)t * MS_TILE;
+            end = min(base + MS_TILE, a.n);
+        }
         if (tid == 0) s.npend = 0;
-        const int32_t v = (int32_t)(base + tid);
+        const int32_t v = base + tid;
         unsigned long long want = 0, got = 0;
         int32_t start = 0, deg = 0;
-        if (v < a.n) {
-            const unsigned long long m = a.bu[a.gid[v]];
+        if (v < end) {
+            const unsigned long long m = a.bu[tg >= 0 ? tg : a.gid[v]];
             if (m) {
                 want = m & ~a.visited[v];
                 if (want) {
@@ -354,11 +408,11 @@ __global__ __launch_bounds__(BFS_THREADS) void k_ms_pull(MsArgs a, int ph) {
         // Stage 2: groups of MS_GROUP lanes scan the remaining lists.
         const uint32_t np = s.npend;
         for (uint32_t i = grp; i < np; i += MS_GROUPS) {
-            const int32_t t = s.pv[i];
-            const int32_t vv = (int32_t)base + t;
+            const int32_t lt = s.pv[i];
+            const int32_t vv = base + lt;
             const int32_t e1 = s.pend_end[i];
             const unsigned long long w = s.pwant[i];
-            unsigned long long g = s.got[t];
+            unsigned long long g = s.got[lt];
             for (int32_t e0 = s.pe[i]; e0 < e1 && g != w; e0 += MS_GROUP) {
                 const int32_t e = e0 + sub;
                 int32_t u = -1;
@@ -371,7 +425,7 @@ __global__ __launch_bounds__(BFS_THREADS) void k_ms_pull(MsArgs a, int ph) {
                 if (mine) ms_record(a, ph, vv, u, mine);
                 g |= group_or(f);
             }
-            if (sub == 0) s.got[t] = g;
+            if (sub == 0) s.got[lt] = g;
         }
         __syncthreads();
         got = s.got[tid];
@@ -391,7 +445,6 @@ __global__ __launch_bounds__(BFS_THREADS) void k_ms_pull(MsArgs a, int ph) {
 // map word never indexes outside a list.
 __global__ __launch_bounds__(BFS_THREADS) void k_ms_heavy(MsArgs a, int ph) {
     __shared__ MsShared s;
-    if (a.dev_base) ph += (int)ld_relaxed(&a.ctr->ph_base);
     const unsigned long long alloc =
         __hip_atomic_load(&a.ctr->heavy_alloc[ph & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const uint32_t H = min((uint32_t)(alloc >> 32), a.heavy_cap);
@@ -432,11 +485,16 @@ __global__ __launch_bounds__(BFS_THREADS) void k_ms_heavy(MsArgs a, int ph) {
 // it, transposes the lanes' bit masks so that lane b holds the set of lanes
 // carrying bit b.  Running totals are kept while the graph stays the same.
 __global__ __launch_bounds__(BFS_THREADS) void k_ms_count(MsArgs a, int ph) {
-    if (a.dev_base) ph += (int)ld_relaxed(&a.ctr->ph_base);
     uint32_t n_q = ld_relaxed(&a.ctr->qlen[(ph + 1) & 3]);
+    // Frontier bitmap of level ph + 1: set entry by entry from the queue while
+    // it is short (the array is all zero: its bits were those of level ph - 1,
+    // cleared below one level ago), else rewritten word by word from the
+    // masks after the counts.
+    const bool dense = n_q > a.qcap || (int64_t)n_q * MS_SPARSE_DIV >= (int64_t)a.n;
     if (n_q > a.qcap) n_q = 0;
-    const int32_t* q = a.queue[(ph + 1) & 1];
+    const int32_t* q = a.queue[(ph + 1) % 3];
     const unsigned long long* nxt = a.front[(ph + 1) % 3];
+    uint32_t* fbm_next = a.fbm[(ph + 1) & 1];
     const int lane = lane_id();
     const int64_t wave = (int64_t)blockIdx.x * BFS_WAVES + threadIdx.x / kWave;
     const int64_t nwaves = (int64_t)gridDim.x * BFS_WAVES;
@@ -455,6 +513,7 @@ __global__ __launch_bounds__(BFS_THREADS) void k_ms_count(MsArgs a, int ph) {
                 g = a.gid[v];
                 bits = nxt[v];
                 deg = a.offs[v + 1] - a.offs[v];
+                if (!dense) atomicOr(&fbm_next[v >> 5], 1u << (v & 31));
             } else {
                 st_relaxed(&a.ctr->corrupt, 1u);
             }
@@ -488,17 +547,24 @@ __global__ __launch_bounds__(BFS_THREADS) void k_ms_count(MsArgs a, int ph) {
         atomic_add_u32(&a.cnt_n[(size_t)accg * 64 + lane], accn);
         atomic_add_u64(&a.cnt_m[(size_t)accg * 64 + lane], accm);
     }
-    // Frontier bitmap of the next level: bit v = (next[v] != 0).  Each wave
-    // writes the two words of its 64 vertices.
     const int64_t words = ((int64_t)a.n + 31) / 32;
-    for (int64_t vb = wave * kWave; vb < a.n; vb += nwaves * kWave) {
-        const int64_t v = vb + lane;
-        const uint64_t m = __ballot(v < a.n && nxt[v] != 0ull);
-        if ((lane & 31) == 0) {
-            const int64_t w = v >> 5;
-            if (w < words) a.fbm[w] = (lane == 0) ? (uint32_t)m : (uint32_t)(m >> 32);
+    if (dense) {
+        // bit v = (next[v] != 0); each wave writes the two words of its 64
+        // vertices.
+        for (int64_t vb = wave * kWave; vb < a.n; vb += nwaves * kWave) {
+            const int64_t v = vb + lane;
+            const uint64_t m = __ballot(v < a.n && nxt[v] != 0ull);
+            if ((lane & 31) == 0) {
+                const int64_t w = v >> 5;
+                if (w < words) fbm_next[w] = (lane == 0) ? (uint32_t)m : (uint32_t)(m >> 32);
+            }
         }
     }
+    // Level ph's bitmap is no longer read (the pull is done); zero it for
+    // level ph + 2.
+    uint32_t* fbm_cur = a.fbm[ph & 1];
+    for (int64_t w = (int64_t)blockIdx.x * BFS_THREADS + threadIdx.x; w < words; w += (int64_t)gridDim.x * BFS_THREADS)
+        fbm_cur[w] = 0u;
 }
 
 // One wave per graph, lane b = slot b.  The slot's level counts decide its
@@ -508,7 +574,6 @@ __global__ __launch_bounds__(BFS_THREADS) void k_ms_count(MsArgs a, int ph) {
 // below V_g / beta.  Also publishes the "found anything" mask per graph and
 // consumes (zeroes) the counts.
 __global__ __launch_bounds__(BFS_THREADS) void k_ms_direction(MsArgs a, int ph) {
-    if (a.dev_base) ph += (int)ld_relaxed(&a.ctr->ph_base);
     const int32_t g = (int32_t)(blockIdx.x * BFS_WAVES + threadIdx.x / kWave);
     if (blockIdx.x == 0 && threadIdx.x == 0) st_relaxed(&a.ctr->any_bu[ph & 1], 0u);
     if (g >= a.G) return;
@@ -540,13 +605,28 @@ __global__ __launch_bounds__(BFS_THREADS) void k_ms_direction(MsArgs a, int ph) 
     if (b == 0) {
         a.bu[g] = bum;
         a.newbits[g] = found;
-        if (bum) st_relaxed(&a.ctr->any_bu[(ph + 1) & 1], 1u);
+        a.hnewbits[g] = found;
+        if (bum) {
+            // List the graph for the next level's pull: one 64-bit atomic
+            // gives both its table entry and its first tile, so the table
+            // is in tile order.
+            st_relaxed(&a.ctr->any_bu[(ph + 1) & 1], 1u);
+            const uint32_t nt = (uint32_t)((Vg + MS_TILE - 1) / MS_TILE);
+            const unsigned long long t = __hip_atomic_fetch_add(
+                &a.ctr->bu_tiles[(ph + 1) & 1], (1ull << 32) | (unsigned long long)nt, __ATOMIC_RELAXED,
+                __HIP_MEMORY_SCOPE_AGENT);
+            const uint32_t e = (uint32_t)(t >> 32);
+            if (e < (uint32_t)a.G) a.bu_list[(size_t)((ph + 1) & 1) * a.G + e] = make_int2(g, (int)(uint32_t)t);
+        }
+    }
+    // The step's last launch: publish the error flags with the masks (the
+    // host reads both after one stream synchronisation, no copies).
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.hctr->overflow = ld_relaxed(&a.ctr->overflow);
+        a.hctr->corrupt = ld_relaxed(&a.ctr->corrupt);
     }
 }
 
-__global__ void k_ms_advance(MsArgs a, int steps) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) a.ctr->ph_base += (uint32_t)steps;
-}
 
 // Per admitted query: graph slot, label base and start level.
 struct MsSeed {
@@ -595,7 +675,7 @@ __global__ __launch_bounds__(256) void k_ms_admit(MsArgs a, int ph, const MsSeed
     if (old == 0ull) {
         const uint32_t pos = atomic_add_u32(&a.ctr->qlen[ph & 3], 1u);
         if (pos < a.qcap)
-            a.queue[ph & 1][pos] = v;
+            a.queue[ph % 3][pos] = v;
         else
             st_relaxed(&a.ctr->overflow, 1u);
     }
@@ -711,8 +791,8 @@ struct b40c_msbfs {
     int32_t n = 0, G = 0;
     int64_t arena_elems = 0;  // per label array (depth and pred each)
     b40c::DeviceBuffer visited, front[3], active, bu, newbits, cnt_n, cnt_m, nf_prev, m_seen, bitq, lbase, qstart, depth,
-        pred, queue[2], counters;
-    b40c::DeviceBuffer seed_buf, rel_buf, heavy, chunk_map, fbm;
+        pred, queue[3], counters;
+    b40c::DeviceBuffer seed_buf, rel_buf, heavy, chunk_map, fbm, bu_list;
     b40c::MsSeed* h_seeds = nullptr;  // pinned staging
     b40c::MsRelease* h_rel = nullptr;  // pinned staging
     int h_rel_cap = 0;
@@ -728,8 +808,10 @@ struct b40c_msbfs {
     int32_t alpha = 14, beta = 24, grid = 1024;
     int32_t cohort_gap = 6;        // levels between admissions while queries run
     int32_t last_admit_ph = -1000;
-    b40c::MsCounters* h_ctr = nullptr;
-    unsigned long long* h_newbits = nullptr;  // pinned [G]
+    b40c::MsCounters* h_ctr = nullptr;      // pinned, written by k_ms_direction (overflow, corrupt)
+    b40c::MsCounters* h_ctr_dev = nullptr;  // its device address
+    unsigned long long* h_newbits_dev = nullptr;
+    unsigned long long* h_newbits = nullptr;  // pinned [G], written by k_ms_direction
     int32_t qmax = 0;
 };
 
@@ -757,10 +839,14 @@ MsArgs ms_args(b40c_msbfs* m) {
     a.qstart = m->qstart.as<int32_t>();
     a.depth_out = m->depth.as<int32_t>();
     a.pred_out = m->pred.as<int32_t>();
-    a.queue[0] = m->queue[0].as<int32_t>();
-    a.queue[1] = m->queue[1].as<int32_t>();
+    for (int i = 0; i < 3; ++i) a.queue[i] = m->queue[i].as<int32_t>();
     a.qcap = m->qcap;
-    a.fbm = m->fbm.as<uint32_t>();
+    const size_t words = ((size_t)m->n + 31) / 32;
+    a.fbm[0] = m->fbm.as<uint32_t>();
+    a.fbm[1] = m->fbm.as<uint32_t>() + words;
+    a.bu_list = m->bu_list.as<int2>();
+    a.hctr = m->h_ctr_dev;
+    a.hnewbits = m->h_newbits_dev;
     a.heavy = m->heavy.as<int2>();
     a.chunk_map = m->chunk_map.as<int32_t>();
     a.heavy_cap = m->heavy_cap;
@@ -773,7 +859,6 @@ MsArgs ms_args(b40c_msbfs* m) {
     a.cta_th = 256;
     a.wave_th = 64;
     a.grid_th = 128;
-    a.dev_base = 0;
     return a;
 }
 
@@ -992,7 +1077,11 @@ b40c_status b40c_msbfs_create(b40c_multigraph* mg, int64_t label_budget_bytes, i
     b40c_status s = device_info(info);
     m->grid = info.num_cus * 4;
     hipError_t e = hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipHostMalloc((void**)&m->h_ctr, sizeof(MsCounters), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&m->h_ctr, sizeof(MsCounters), hipHostMallocCoherent);
+    if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&m->h_ctr_dev, m->h_ctr, 0);
+    if (e == hipSuccess)
+        e = hipHostMalloc((void**)&m->h_newbits, (size_t)std::max(m->G, 1) * 8, hipHostMallocCoherent);
+    if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&m->h_newbits_dev, m->h_newbits, 0);
     if (e != hipSuccess) s = fail(B40C_ERR_HIP, "b40c_msbfs_create: %s", hipGetErrorString(e));
     const size_t N = (size_t)m->n;
     if (s == B40C_OK) s = m->visited.alloc(N * 8);
@@ -1009,21 +1098,22 @@ b40c_status b40c_msbfs_create(b40c_multigraph* mg, int64_t label_budget_bytes, i
     if (s == B40C_OK) s = m->qstart.alloc((size_t)max_queries * 4);
     if (s == B40C_OK) s = m->depth.alloc((size_t)m->arena_elems * 4);
     if (s == B40C_OK) s = m->pred.alloc((size_t)m->arena_elems * 4);
-    if (s == B40C_OK) s = m->queue[0].alloc(N * 4);
-    if (s == B40C_OK) s = m->queue[1].alloc(N * 4);
+    for (int i = 0; i < 3 && s == B40C_OK; ++i) s = m->queue[i].alloc(N * 4);
+    if (s == B40C_OK) s = m->bu_list.alloc((size_t)m->G * 2 * sizeof(int2));
     if (s == B40C_OK) s = m->counters.alloc(sizeof(MsCounters));
     m->heavy_cap = (uint32_t)m->n;
     m->chunk_cap = (uint32_t)std::min<int64_t>(mg->packed->edges / HEAVY_CHUNK + (int64_t)m->n + 64, 0x7FFFFFFFll);
     if (s == B40C_OK) s = m->heavy.alloc((size_t)m->heavy_cap * sizeof(int2));
     if (s == B40C_OK) s = m->chunk_map.alloc((size_t)m->chunk_cap * 4);
-    if (s == B40C_OK) s = m->fbm.alloc(((size_t)m->n + 31) / 32 * 4 + 4);
+    if (s == B40C_OK) s = m->fbm.alloc(((size_t)m->n + 31) / 32 * 4 * 2 + 4);
     if (s != B40C_OK) {
         b40c_msbfs_destroy(m);
         return s;
     }
     m->arena.init(m->arena_elems);
     m->h_active.assign(m->G, 0ull);
-    B40C_HIP(hipHostMalloc((void**)&m->h_newbits, (size_t)m->G * 8, hipHostMallocDefault));
+    std::memset(m->h_ctr, 0, sizeof(MsCounters));
+    std::memset(m->h_newbits, 0, (size_t)m->G * 8);
     B40C_HIP(hipMemsetAsync(m->visited.ptr, 0, m->visited.bytes, m->stream));
     for (int i = 0; i < 3; ++i) B40C_HIP(hipMemsetAsync(m->front[i].ptr, 0, m->front[i].bytes, m->stream));
     B40C_HIP(hipMemsetAsync(m->newbits.ptr, 0, m->newbits.bytes, m->stream));
@@ -1088,9 +1178,7 @@ b40c_status b40c_msbfs_step(b40c_msbfs* m, int32_t* running, int32_t* waiting) {
         hipLaunchKernelGGL(k_ms_direction, dim3((m->G + BFS_WAVES - 1) / BFS_WAVES), dim3(BFS_THREADS), 0, m->stream,
                            a, m->ph);
         B40C_HIP(hipGetLastError());
-        B40C_HIP(hipMemcpyAsync(m->h_ctr, m->counters.ptr, sizeof(MsCounters), hipMemcpyDeviceToHost, m->stream));
-        B40C_HIP(hipMemcpyAsync(m->h_newbits, m->newbits.ptr, (size_t)m->G * 8, hipMemcpyDeviceToHost,
-                                m->stream));
+        // k_ms_direction has written h_ctr's flags and h_newbits.
         B40C_HIP(hipStreamSynchronize(m->stream));
         if (m->h_ctr->overflow || m->h_ctr->corrupt)
             return fail(B40C_ERR_HIP, "batched traversal: frontier overflow or corrupt frontier");
@@ -1163,8 +1251,8 @@ void b40c_msbfs_destroy(b40c_msbfs* m) {
     m->qstart.release();
     m->depth.release();
     m->pred.release();
-    m->queue[0].release();
-    m->queue[1].release();
+    for (int i = 0; i < 3; ++i) m->queue[i].release();
+    m->bu_list.release();
     m->heavy.release();
     m->chunk_map.release();
     m->fbm.release();

@@ -159,3 +159,41 @@ def test_admission_policy_does_not_change_labels(mods, orc, pool):
         np.testing.assert_array_equal(depth, ref)
     with pytest.raises(Exception):
         batch.MultiSourceBFS(mg, label_budget_bytes=1 << 20).set_admission(0)
+
+
+@pytest.mark.parametrize("gap", [1, 6])
+def test_staggered_sessions_with_idle_gaps(mods, orc, gap):
+    """Long continuous run: one query arrives at every level boundary, the
+    engine drains to idle between sessions and restarts, slots are reused.
+    Per-level state that is cleared from the frontier queues (mask arrays,
+    frontier bitmaps, the pull's graph table) must never leak from one
+    query, level or session into another."""
+    bfs, batch = mods
+    specs = [("rmat", 15, 61), ("rmat", 9, 62), ("lattice", 10, 63), ("rmat", 12, 64)]
+    graphs, host = [], []
+    for kind, p, seed in specs:
+        g = bfs.Graph.rmat(p, 16, 0.57, 0.19, 0.19, seed) if kind == "rmat" else bfs.Graph.lattice3d(p, seed)
+        graphs.append(g)
+        host.append(g.to_host())
+    mg = batch.MultiGraph(graphs)
+    eng = batch.MultiSourceBFS(mg, label_budget_bytes=1 << 28)
+    eng.set_admission(gap)
+    rng = np.random.default_rng(70 + gap)
+    ids, out = {}, {}
+    for session in range(3):
+        arrivals = [int(rng.integers(0, len(specs))) for _ in range(40)]
+        for k, g in enumerate(arrivals):
+            s = sources(orc, host, g, 1, 1000 * session + k)[0]
+            ids[eng.submit(g, s)] = (g, s)
+            eng.step()
+            for q in eng.poll():
+                out[q] = eng.result(q, ids[q][0])
+        eng.run()  # drain: the engine goes idle before the next session
+        for q in eng.poll():
+            out[q] = eng.result(q, ids[q][0])
+    assert sorted(out) == sorted(ids)
+    for q, (depth, pred, sd) in out.items():
+        g, s = ids[q]
+        check_query(orc, host, g, s, depth, pred, sd)
+    eng.close()
+    mg.close()
