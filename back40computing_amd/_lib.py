@@ -39,6 +39,11 @@ MODE_DEFAULT = 0
 MODE_FUSED = 1
 MODE_ITERATIVE = 2
 
+# graph file formats (include/b40c_mi355x.h B40C_FILE_*)
+FILE_MARKET = 0
+FILE_METIS = 1
+FILE_DIMACS = 2
+
 
 class B40CError(RuntimeError):
     """A non-OK b40c_status; ``.status`` holds the code."""
@@ -87,6 +92,13 @@ SIGNATURES = {
     "b40c_graph_info": (_i32, [_vp, _p_i32, _p_i64]),
     "b40c_graph_copy_to_host": (_i32, [_vp, _vp, _vp]),
     "b40c_graph_destroy": (None, [_vp]),
+    "b40c_edge_list_read": (_i32, [ctypes.c_char_p, _i32, _i32, _i32, _pp]),
+    "b40c_edge_list_info": (_i32, [_vp, _p_i32, _p_i64, _p_i32]),
+    "b40c_edge_list_copy": (_i32, [_vp, _vp, _vp]),
+    "b40c_graph_from_edge_list": (_i32, [_vp, _pp]),
+    "b40c_edge_list_destroy": (None, [_vp]),
+    "b40c_graph_read": (_i32, [ctypes.c_char_p, _i32, _i32, _i32, _pp]),
+    "b40c_graph_write_metis": (_i32, [_vp, ctypes.c_char_p]),
     "b40c_radix_sort_u64": (_i32, [_vp, _vp, _i64, _i32, _i32, _p_i32]),
     "b40c_radix_sort_pairs_u32": (_i32, [_vp, _vp, _vp, _vp, _i64, _i32, _i32, _p_i32]),
     "b40c_device_alloc": (_i32, [_pp, _i64]),

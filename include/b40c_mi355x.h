@@ -111,6 +111,39 @@ b40c_status b40c_graph_copy_to_host(const b40c_graph* g, int32_t* row_offsets, i
 /* Graphs are reference counted: problems hold a reference. */
 void b40c_graph_destroy(b40c_graph* g);
 
+/* ---- graph files (test_bfs.cu:976-1037: graph types dimacs, metis, market) */
+#define B40C_FILE_MARKET 0 /* MatrixMarket coordinate (b40c/graph/builder/market.cuh) */
+#define B40C_FILE_METIS  1 /* METIS adjacency lists (b40c/graph/builder/metis.cuh)    */
+#define B40C_FILE_DIMACS 2 /* DIMACS "p"/"a" lines (b40c/graph/builder/dimacs.cuh)   */
+typedef struct b40c_edge_list b40c_edge_list;
+/* Parses a graph file on the host (path NULL: stdin), in the reference
+ * readers' edge order: ReadMarketStream (market.cuh:50-158), ReadMetisStream
+ * (metis.cuh:52-158), ReadDimacsStream (dimacs.cuh:54-220).  undirected adds
+ * each edge's reverse (MatrixMarket, DIMACS); splice > 0 reads DIMACS files
+ * path.0 .. path.<splice-1> merged by row (BuildDimacsGraph, dimacs.cuh:230-270).
+ * Ids outside [1, n] are an error (B40C_ERR_INVALID_ARG); weighted METIS
+ * formats are B40C_ERR_UNSUPPORTED.  Needs no GPU. */
+b40c_status b40c_edge_list_read(const char* path, int32_t format, int32_t undirected, int32_t splice,
+                                b40c_edge_list** out);
+/* Vertex count, directed edge count, and 1 when the list is a CSR (METIS)
+ * rather than a COO list. */
+b40c_status b40c_edge_list_info(const b40c_edge_list* el, int32_t* nodes, int64_t* edges, int32_t* is_csr);
+/* Copies the COO rows (edges entries, file order) or CSR offsets (nodes + 1)
+ * and the columns; either pointer may be NULL. */
+b40c_status b40c_edge_list_copy(const b40c_edge_list* el, int32_t* rows_or_offsets, int32_t* cols);
+/* Device CSR of a parsed file: COO through the GPU stable sort by row
+ * (CsrGraph::FromCoo, csr_graph.cuh:108-152), METIS uploaded as read. */
+b40c_status b40c_graph_from_edge_list(const b40c_edge_list* el, b40c_graph** out);
+void b40c_edge_list_destroy(b40c_edge_list* el);
+/* Read + build in one call (BuildMarketGraph market.cuh:172, BuildMetisGraph
+ * metis.cuh:184, BuildDimacsGraph dimacs.cuh:230, then the device CSR). */
+b40c_status b40c_graph_read(const char* path, int32_t format, int32_t undirected, int32_t splice,
+                            b40c_graph** out);
+/* Writes a graph as METIS text, "n edges" then one line of 1-based
+ * neighbours per vertex (WriteMetisStream, metis.cuh:220-245); path NULL:
+ * stdout. */
+b40c_status b40c_graph_write_metis(const b40c_graph* g, const char* path);
+
 /* ---- radix sort (b40c/radix_sort/enactor.cuh:153-266) ------------------- */
 /* Stable LSD sort of device keys over bits [begin_bit, end_bit) with 8-bit
  * digits (upsweep -> spine -> downsweep per digit place).  d_keys[0] holds the

@@ -7,6 +7,8 @@
 // reference-side caller would after relinking.
 //
 // usage: test_bfs rmat <scale> [edge_factor] | grid3d <width>
+//                 | market <file> | metis <file> | dimacs <file>
+//                 [--undirected] [--splice=<n>]
 //                 [--strategy=0..5] [--mode=default|fused|iterative]
 //                 [--i=<sources>] [--src=<vertex>] [--seed=<n>]
 //                 [--queue-sizing=<f>]
@@ -21,6 +23,7 @@
 #include <vector>
 
 #include "b40c/bfs.hpp"
+#include "b40c/graph_builder.hpp"
 
 extern "C" {
 int32_t orc_bfs(int32_t n, const int32_t* offsets, const int32_t* cols, int32_t src, int32_t* depth, int32_t* pred);
@@ -44,7 +47,12 @@ struct Options {
     int src = -1;
     unsigned long long seed = 1;
     double queue_sizing = 1.3;
+    std::string file;
+    bool undirected = false;  // file graphs: add a back edge per edge (test_bfs.cu:182-185)
+    int splice = 0;
 };
+
+bool file_kind(const std::string& k) { return k == "market" || k == "metis" || k == "dimacs"; }
 
 bool parse(int argc, char** argv, Options& o) {
     int pos = 0;
@@ -56,14 +64,17 @@ bool parse(int argc, char** argv, Options& o) {
         else if (!strncmp(a, "--src=", 6)) o.src = atoi(a + 6);
         else if (!strncmp(a, "--seed=", 7)) o.seed = strtoull(a + 7, nullptr, 10);
         else if (!strncmp(a, "--queue-sizing=", 15)) o.queue_sizing = atof(a + 15);
+        else if (!strcmp(a, "--undirected")) o.undirected = true;
+        else if (!strncmp(a, "--splice=", 9)) o.splice = atoi(a + 9);
         else if (a[0] == '-') return false;
         else if (pos == 0) { o.kind = a; ++pos; }
         else if (o.kind == "rmat" && pos == 1) { o.scale = atoi(a); ++pos; }
         else if (o.kind == "rmat" && pos == 2) { o.edge_factor = atoi(a); ++pos; }
         else if (o.kind == "grid3d" && pos == 1) { o.width = atoi(a); ++pos; }
+        else if (file_kind(o.kind) && pos == 1) { o.file = a; ++pos; }
         else return false;
     }
-    return o.kind == "rmat" || o.kind == "grid3d";
+    return o.kind == "rmat" || o.kind == "grid3d" || (file_kind(o.kind) && !o.file.empty());
 }
 
 // The enactor classes are distinct types (as in the reference); run the
@@ -144,14 +155,20 @@ int run(const Options& o, b40c_graph* g, const std::vector<int32_t>& offs, const
 int main(int argc, char** argv) {
     Options o;
     if (!parse(argc, argv, o)) {
-        fprintf(stderr, "usage: %s rmat <scale> [edge_factor] | grid3d <width> [--strategy=0..5] "
+        fprintf(stderr, "usage: %s rmat <scale> [edge_factor] | grid3d <width> | market <file> | metis <file> "
+                        "| dimacs <file> [--undirected] [--splice=N] [--strategy=0..5] "
                         "[--mode=default|fused|iterative] [--i=N] [--src=V] [--seed=S] [--queue-sizing=F]\n",
                 argv[0]);
         return 2;
     }
     b40c_graph* g = nullptr;
-    b40c_status s = o.kind == "rmat" ? b40c_graph_build_rmat(o.scale, o.edge_factor, 0.57, 0.19, 0.19, o.seed, &g)
-                                     : b40c_graph_build_lattice3d(o.width, o.seed, nullptr, &g);
+    namespace builder = b40c::graph::builder;
+    b40c_status s;
+    if (o.kind == "rmat") s = b40c_graph_build_rmat(o.scale, o.edge_factor, 0.57, 0.19, 0.19, o.seed, &g);
+    else if (o.kind == "grid3d") s = b40c_graph_build_lattice3d(o.width, o.seed, nullptr, &g);
+    else if (o.kind == "market") s = builder::BuildMarketGraph(o.file.c_str(), &g, o.undirected);
+    else if (o.kind == "metis") s = builder::BuildMetisGraph(o.file.c_str(), &g);
+    else s = builder::BuildDimacsGraph(o.file.c_str(), &g, o.undirected, o.splice);
     if (s != B40C_OK) {
         fprintf(stderr, "graph build: %s\n", b40c_last_error_string());
         return 2;

@@ -45,3 +45,50 @@ def test_cpp_sort_driver():
     print(r.stdout[-2000:])
     assert r.returncode == 0, r.stdout + r.stderr
     assert "all CORRECT" in r.stdout
+
+
+def _grid_edges(w):
+    """Undirected edges (1-based) of a w x w grid, each listed once."""
+    out = []
+    for r in range(w):
+        for c in range(w):
+            v = r * w + c + 1
+            if c + 1 < w:
+                out.append((v, v + 1))
+            if r + 1 < w:
+                out.append((v, v + w))
+    return out
+
+
+@pytest.mark.parametrize("kind", ["market", "metis", "dimacs"])
+def test_cpp_driver_file_graphs(tmp_path, kind):
+    """The driver's file graph types (test_bfs.cu:976-1037) through the
+    facade's builders (include/b40c/graph_builder.hpp)."""
+    if not os.path.exists(DRIVER):
+        pytest.fail("C++ driver not built (run __graft_entry__.build())")
+    w = 60
+    n, edges = w * w, _grid_edges(w)
+    path = str(tmp_path / ("grid." + kind))
+    with open(path, "w") as f:
+        if kind == "market":
+            f.write("%%%%MatrixMarket matrix coordinate pattern symmetric\n%d %d %d\n" % (n, n, len(edges)))
+            f.write("".join("%d %d\n" % e for e in edges))
+        elif kind == "dimacs":
+            f.write("p sp %d %d\n" % (n, len(edges)))
+            f.write("".join("a %d %d 1\n" % e for e in edges))
+        else:
+            adj = [[] for _ in range(n)]
+            for a, b in edges:
+                adj[a - 1].append(b)
+                adj[b - 1].append(a)
+            f.write("%d %d\n" % (n, len(edges)))
+            f.write("".join(" ".join(map(str, sorted(a))) + "\n" for a in adj))
+    extra = [] if kind == "metis" else ["--undirected"]
+    for strategy in ("--strategy=0", "--strategy=5"):
+        print("driver file case:", kind, strategy, flush=True)
+        r = subprocess.run([DRIVER, kind, path, *extra, strategy, "--i=3", "--seed=2"], capture_output=True,
+                           text=True, timeout=120)
+        print(r.stdout[-1500:])
+        assert r.returncode == 0, r.stdout + r.stderr
+        assert "graph %s: nodes=%d edges=%d" % (kind, n, 2 * len(edges)) in r.stdout
+        assert "failures=0" in r.stdout
