@@ -301,6 +301,73 @@ def bench_served(b4, info, duration_s=1.5):
     return out
 
 
+def _ingest_one(graph_io, path, fmt, off, cols, write_s=None, reps=2):
+    """Best of `reps` reads of one file: host parse (b40c_edge_list_read)
+    and device CSR build (b40c_graph_from_edge_list); the CSR must be the
+    graph the file was written from."""
+    size = os.path.getsize(path)
+    best = None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        el = graph_io.EdgeList(path, fmt)
+        t1 = time.perf_counter()
+        g = el.to_graph()
+        t2 = time.perf_counter()
+        if best is None or t2 - t0 < best[0] + best[1]:
+            best = (t1 - t0, t2 - t1, el.edges)
+        el.close()
+    o2, c2 = g.to_host()
+    parse_s, build_s, m = best
+    out = {"file_mb": round(size / 1e6, 1), "edges": m, "parse_s": round(parse_s, 4), "build_s": round(build_s, 4),
+           "parse_mb_per_s": round(size / 1e6 / parse_s, 1),
+           "medges_per_s": round(m / (parse_s + build_s) / 1e6, 1),
+           "csr_identical": bool(np.array_equal(o2, off) and np.array_equal(c2, cols))}
+    if write_s is not None:
+        out["write_s"] = round(write_s, 4)
+    g.close()
+    return out
+
+
+def bench_ingest(scale=20, text_scale=16):
+    """Graph files (MatrixMarket, METIS, DIMACS; test_bfs.cu:976-1037): an
+    RMAT-`scale` graph written as METIS by the library's writer, an
+    RMAT-`text_scale` graph written as MatrixMarket and DIMACS text (one
+    directed entry per CSR edge, rows in order), each read back: parse on the
+    host cores, device CSR built on the GPU, checked identical to the
+    graph written."""
+    import tempfile
+    from back40computing_amd import graph_io
+    from back40computing_amd.bfs import Graph
+    out = {"threads": min(16, os.cpu_count() or 1)}
+    with tempfile.TemporaryDirectory() as d:
+        g = Graph.rmat(scale, 16, 0.57, 0.19, 0.19, 3)
+        off, cols = g.to_host()
+        path = os.path.join(d, "rmat.graph")
+        t = time.perf_counter()
+        graph_io.write_metis(g, path)
+        out["metis_rmat%d" % scale] = _ingest_one(graph_io, path, "metis", off, cols, time.perf_counter() - t)
+        g.close()
+        os.remove(path)
+        g = Graph.rmat(text_scale, 16, 0.57, 0.19, 0.19, 4)
+        off, cols = g.to_host()
+        rows = (np.repeat(np.arange(g.nodes), np.diff(off)) + 1).tolist()
+        c1 = (cols.astype(np.int64) + 1).tolist()
+        mpath = os.path.join(d, "rmat.mtx")
+        with open(mpath, "w") as f:
+            f.write("%%%%MatrixMarket matrix coordinate pattern general\n%d %d %d\n" % (g.nodes, g.nodes, g.edges))
+            f.write("\n".join("%d %d" % rc for rc in zip(c1, rows)))
+            f.write("\n")
+        out["market_rmat%d" % text_scale] = _ingest_one(graph_io, mpath, "market", off, cols)
+        dpath = os.path.join(d, "rmat.gr")
+        with open(dpath, "w") as f:
+            f.write("p sp %d %d\n" % (g.nodes, g.edges))
+            f.write("\n".join("a %d %d 1" % rc for rc in zip(rows, c1)))
+            f.write("\n")
+        out["dimacs_rmat%d" % text_scale] = _ingest_one(graph_io, dpath, "dimacs", off, cols)
+        g.close()
+    return out
+
+
 def pick_sources_cached(off, seed, count):
     from back40computing_amd.sources import pick_sources
     return pick_sources(off, seed, count)
@@ -393,6 +460,7 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched 1,024-query workload")
     ap.add_argument("--no-served", action="store_true", help="skip the served Poisson workload")
     ap.add_argument("--no-lattice", action="store_true", help="skip the 200^3 lattice (high-diameter) workload")
+    ap.add_argument("--no-ingest", action="store_true", help="skip the graph-file read/build workload")
     ap.add_argument("--only", choices=["batched", "served"], help=argparse.SUPPRESS)
     ap.add_argument("--probe", action="store_true",
                     help="counter-collection run: a 1 GiB copy plus --steps searches, no JSON line")
@@ -474,6 +542,14 @@ def main():
         except b4.B40CError as ex:
             lattice = {"error": str(ex)}
             log("lattice workload failed:", ex)
+    ingest = None
+    if not args.no_ingest:
+        try:
+            ingest = bench_ingest()
+            log("ingest", json.dumps(ingest))
+        except b4.B40CError as ex:
+            ingest = {"error": str(ex)}
+            log("ingest workload failed:", ex)
     # The batched and served workloads run in child processes: they build
     # their own graph pools, and whatever happens there cannot take the
     # headline measurement down with it.
@@ -529,6 +605,7 @@ def main():
         "lattice": lattice,
         "batched": batched,
         "served": served,
+        "ingest": ingest,
         "detail": {"gteps_enact_only": round(head["gteps_enact_only"], 3),
                    "ms_per_search_enact": round(head["ms_per_search_enact"], 4),
                    "b_alg_per_search": head["b_alg_per_search"],
