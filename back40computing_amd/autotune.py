@@ -34,7 +34,7 @@ SPACE = {
     "cta_gather_threshold": [128, 256, 512, 1024, 4096],
     "wave_gather_threshold": [16, 32, 64, 128],
     "grid_size": [256, 512, 1024, 2048, 4096],
-    "levels_per_sync": [2, 4, 6, 7, 8, 12, 16],
+    "levels_per_sync": [2, 4, 6, 7, 8, 12, 16, 32, 64],
     "saturation_quit": [64, 256, 512, 2048],
     "bottom_up_grid_size": [1024, 2048, 4096],
 }

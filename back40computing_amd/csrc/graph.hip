@@ -39,6 +39,25 @@ void graph_release(b40c_graph* g) {
 
 namespace {
 __global__ void k_check_symmetric(const int32_t* offs, const int32_t* cols, int32_t n, uint32_t* flags);
+__global__ void k_max_degree(const int32_t* offs, int32_t n, uint32_t* out);
+}
+
+b40c_status graph_max_degree(b40c_graph* g, int32_t& max_degree) {
+    int known = g->max_degree.load();
+    if (known < 0) {
+        DeviceBuffer out;
+        B40C_TRY(out.alloc(sizeof(uint32_t)));
+        B40C_HIP(hipMemset(out.ptr, 0, sizeof(uint32_t)));
+        hipLaunchKernelGGL(k_max_degree, dim3(1024), dim3(256), 0, nullptr, g->d_offsets(), g->nodes,
+                           out.as<uint32_t>());
+        B40C_HIP(hipGetLastError());
+        uint32_t h = 0;
+        B40C_HIP(hipMemcpy(&h, out.ptr, sizeof(h), hipMemcpyDeviceToHost));
+        known = (int)h;
+        g->max_degree.store(known);
+    }
+    max_degree = known;
+    return B40C_OK;
 }
 
 b40c_status graph_symmetric(b40c_graph* g, bool& symmetric) {
@@ -233,6 +252,16 @@ __global__ __launch_bounds__(GB_THREADS) void k_check_symmetric(const int32_t* o
     }
     if (bad_order) atomic_add_u32(&flags[0], bad_order);
     if (missing) atomic_add_u32(&flags[1], missing);
+}
+
+// Largest out-degree: grid-stride over vertices, wave maximum, one atomic
+// per wave.
+__global__ __launch_bounds__(GB_THREADS) void k_max_degree(const int32_t* offs, int32_t n, uint32_t* out) {
+    uint32_t m = 0;
+    for (int64_t v = (int64_t)blockIdx.x * GB_THREADS + threadIdx.x; v < n; v += (int64_t)gridDim.x * GB_THREADS)
+        m = max(m, (uint32_t)(offs[v + 1] - offs[v]));
+    for (int d = kWave / 2; d >= 1; d >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, d, kWave));
+    if (threadIdx.x % kWave == 0 && m) atomicMax(out, m);
 }
 
 __global__ void k_u32_to_i32_copy(const uint32_t* src, int32_t* dst, int64_t n) {

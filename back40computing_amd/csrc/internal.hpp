@@ -56,6 +56,8 @@ struct b40c_graph {
     // 1: every edge (u,v) has its reverse (v,u); 0: not (or not provable);
     // -1: not yet checked.  Bottom-up BFS steps are only valid when 1.
     std::atomic<int> symmetric{-1};
+    // Largest out-degree, or -1 while not yet computed (graph_max_degree).
+    std::atomic<int> max_degree{-1};
     const int32_t* d_offsets() const { return offsets.as<int32_t>(); }
     const int32_t* d_cols() const { return cols.as<int32_t>(); }
 };
@@ -65,4 +67,5 @@ void graph_retain(b40c_graph* g);
 void graph_release(b40c_graph* g);
 // Resolves g->symmetric (runs a GPU check the first time it is unknown).
 b40c_status graph_symmetric(b40c_graph* g, bool& symmetric);
+b40c_status graph_max_degree(b40c_graph* g, int32_t& max_degree);
 }  // namespace b40c
