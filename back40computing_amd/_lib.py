@@ -44,6 +44,10 @@ FILE_MARKET = 0
 FILE_METIS = 1
 FILE_DIMACS = 2
 
+# b40c_graph_from_coo_unique flags
+COO_DROP_SELF_LOOPS = 1
+COO_SYMMETRIZE = 2
+
 
 class B40CError(RuntimeError):
     """A non-OK b40c_status; ``.status`` holds the code."""
@@ -87,6 +91,9 @@ SIGNATURES = {
     "b40c_device_info": (_i32, [_p_i32, ctypes.c_char_p, _i32]),
     "b40c_graph_from_host_csr": (_i32, [_i32, _i64, _vp, _vp, _pp]),
     "b40c_graph_from_coo": (_i32, [_i32, _i64, _vp, _vp, _pp]),
+    "b40c_graph_from_coo_unique": (_i32, [_i32, _i64, _vp, _vp, _i32, _pp]),
+    "b40c_consecutive_removal_u64": (_i32, [_vp, _vp, _vp, _vp, _i64, _p_i64]),
+    "b40c_scan_u32": (_i32, [_vp, _vp, _i64, _i32, ctypes.POINTER(ctypes.c_uint64)]),
     "b40c_graph_build_rmat": (_i32, [_i32, _i32, _d, _d, _d, _u64, _pp]),
     "b40c_graph_build_lattice3d": (_i32, [_i32, _u64, _vp, _pp]),
     "b40c_graph_info": (_i32, [_vp, _p_i32, _p_i64]),

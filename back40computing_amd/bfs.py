@@ -83,6 +83,17 @@ class Graph:
         return cls(h)
 
     @classmethod
+    def from_coo_unique(cls, nodes, rows, cols, drop_self_loops=False, symmetrize=False):
+        """CSR with sorted, duplicate-free rows built on the GPU (sort, consecutive
+        removal, offsets); optionally without self-loops and symmetrized."""
+        r = np.ascontiguousarray(rows, np.int32)
+        c = np.ascontiguousarray(cols, np.int32)
+        flags = (_lib.COO_DROP_SELF_LOOPS if drop_self_loops else 0) | (_lib.COO_SYMMETRIZE if symmetrize else 0)
+        h = ctypes.c_void_p()
+        check(lib().b40c_graph_from_coo_unique(nodes, r.size, ptr(r), ptr(c), flags, ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
     def rmat(cls, scale, edge_factor=16, a=0.57, b=0.19, c=0.19, seed=0):
         h = ctypes.c_void_p()
         check(lib().b40c_graph_build_rmat(scale, edge_factor, a, b, c, seed, ctypes.byref(h)))

@@ -317,6 +317,59 @@ b40c_status csr_from_sorted_keys(const uint64_t* keys, int64_t nkeys, int shift,
     return B40C_OK;
 }
 
+// COO edge e -> packed key (row << shift) | col, plus its reverse at m + e
+// when symmetrizing; dropped self-loops become the sentinel.
+__global__ __launch_bounds__(GB_THREADS) void k_pack_coo(const int32_t* rows, const int32_t* cols, int64_t m,
+                                                         int shift, int drop_loops, int symmetrize, uint64_t sentinel,
+                                                         uint64_t* keys) {
+    for (int64_t e = (int64_t)blockIdx.x * GB_THREADS + threadIdx.x; e < m; e += (int64_t)gridDim.x * GB_THREADS) {
+        const uint64_t r = (uint32_t)rows[e], c = (uint32_t)cols[e];
+        const bool loop = drop_loops && r == c;
+        keys[e] = loop ? sentinel : ((r << shift) | c);
+        if (symmetrize) keys[m + e] = loop ? sentinel : ((c << shift) | r);
+    }
+}
+
+// COO -> CSR with sorted, unique columns per row: pack, GPU radix sort of the
+// packed keys, consecutive removal, row offsets (the steps around the sort
+// that the reference does with consecutive_removal::Enactor::Trim and
+// scan::Enactor::Scan).
+b40c_status build_from_coo_unique(int32_t n, int64_t m, const int32_t* rows, const int32_t* cols, int32_t flags,
+                                  b40c_graph* g) {
+    if (n < 1 || m < 0 || (m > 0 && (!rows || !cols)) || (flags & ~(B40C_COO_DROP_SELF_LOOPS | B40C_COO_SYMMETRIZE)))
+        return fail(B40C_ERR_INVALID_ARG, "from_coo_unique: bad arguments");
+    for (int64_t e = 0; e < m; ++e)
+        if (rows[e] < 0 || rows[e] >= n || cols[e] < 0 || cols[e] >= n)
+            return fail(B40C_ERR_INVALID_ARG, "from_coo_unique: edge %lld (%d,%d) out of range [0,%d)",
+                        (long long)e, rows[e], cols[e], n);
+    const bool sym = (flags & B40C_COO_SYMMETRIZE) != 0;
+    const int64_t nkeys = sym ? 2 * m : m;
+    if (nkeys > 0xFFFFFFFFll) return fail(B40C_ERR_UNSUPPORTED, "from_coo_unique: %lld keys exceed the sorter",
+                                          (long long)nkeys);
+    hipStream_t stream = nullptr;
+    const int shift = std::max(1, ceil_log2_u64((uint64_t)n));
+    const uint64_t sentinel = 1ull << (2 * shift);  // above every real key
+    DeviceBuffer r_d, c_d, k0, k1;
+    B40C_TRY(r_d.alloc((size_t)std::max<int64_t>(m, 1) * 4));
+    B40C_TRY(c_d.alloc((size_t)std::max<int64_t>(m, 1) * 4));
+    B40C_TRY(k0.alloc((size_t)std::max<int64_t>(nkeys, 1) * 8));
+    B40C_TRY(k1.alloc((size_t)std::max<int64_t>(nkeys, 1) * 8));
+    if (m > 0) {
+        B40C_HIP(hipMemcpy(r_d.ptr, rows, (size_t)m * 4, hipMemcpyHostToDevice));
+        B40C_HIP(hipMemcpy(c_d.ptr, cols, (size_t)m * 4, hipMemcpyHostToDevice));
+        hipLaunchKernelGGL(k_pack_coo, dim3(grid_for(m)), dim3(GB_THREADS), 0, stream, r_d.as<int32_t>(),
+                           c_d.as<int32_t>(), m, shift, (flags & B40C_COO_DROP_SELF_LOOPS) ? 1 : 0, sym ? 1 : 0,
+                           sentinel, k0.as<uint64_t>());
+        B40C_HIP(hipGetLastError());
+    }
+    int32_t sel = 0;
+    if (nkeys > 0) B40C_TRY(radix_sort_u64(k0.as<uint64_t>(), k1.as<uint64_t>(), nkeys, 0, 2 * shift + 1, &sel, stream));
+    const uint64_t* sorted = sel ? k1.as<uint64_t>() : k0.as<uint64_t>();
+    B40C_TRY(csr_from_sorted_keys(sorted, nkeys, shift, sentinel, n, g, stream));
+    if (sym) g->symmetric.store(1);
+    return B40C_OK;
+}
+
 b40c_status build_rmat(int scale, int edge_factor, double a, double b, double c, uint64_t seed, b40c_graph* g) {
     if (scale < 1 || scale > 30 || edge_factor < 1 || a < 0 || b < 0 || c < 0 || a + b + c > 1.0)
         return fail(B40C_ERR_INVALID_ARG, "rmat: bad parameters (scale=%d ef=%d a=%g b=%g c=%g)", scale, edge_factor,
@@ -449,6 +502,19 @@ b40c_status b40c_graph_from_coo(int32_t nodes, int64_t edges, const int32_t* row
     if (!out) return fail(B40C_ERR_INVALID_ARG, "b40c_graph_from_coo: null out");
     b40c_graph* g = new b40c_graph();
     b40c_status s = b40c::build_from_coo(nodes, edges, rows, cols, g);
+    if (s != B40C_OK) {
+        delete g;
+        return s;
+    }
+    *out = g;
+    return B40C_OK;
+}
+
+b40c_status b40c_graph_from_coo_unique(int32_t nodes, int64_t edges, const int32_t* rows, const int32_t* cols,
+                                       int32_t flags, b40c_graph** out) {
+    if (!out) return fail(B40C_ERR_INVALID_ARG, "b40c_graph_from_coo_unique: null out");
+    b40c_graph* g = new b40c_graph();
+    b40c_status s = b40c::build_from_coo_unique(nodes, edges, rows, cols, flags, g);
     if (s != B40C_OK) {
         delete g;
         return s;

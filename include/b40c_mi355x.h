@@ -95,6 +95,17 @@ b40c_status b40c_graph_from_host_csr(int32_t nodes, int64_t edges, const int32_t
  * done with the GPU LSD radix sort instead of std::stable_sort.  Host COO in. */
 b40c_status b40c_graph_from_coo(int32_t nodes, int64_t edges, const int32_t* rows, const int32_t* cols,
                                 b40c_graph** out);
+/* GPU COO -> CSR with sorted, duplicate-free columns per row: packed
+ * (row, col) keys, GPU radix sort, consecutive removal and row offsets (the
+ * steps the reference builds from consecutive_removal::Enactor::Trim,
+ * consecutive_removal/enactor.cuh:119-126, and scan::Enactor::Scan,
+ * scan/enactor.cuh:122-136).  flags: B40C_COO_DROP_SELF_LOOPS drops (v, v),
+ * B40C_COO_SYMMETRIZE adds every edge's reverse (the graph is then marked
+ * symmetric).  Host COO in. */
+#define B40C_COO_DROP_SELF_LOOPS 1
+#define B40C_COO_SYMMETRIZE      2
+b40c_status b40c_graph_from_coo_unique(int32_t nodes, int64_t edges, const int32_t* rows, const int32_t* cols,
+                                       int32_t flags, b40c_graph** out);
 /* Synthetic RMAT graph built on the GPU: 2^scale vertices, edge_factor*2^scale
  * sampled edges, quadrant probabilities (a,b,c,1-a-b-c), symmetrized,
  * self-loops and duplicates removed, int32 CSR from a radix sort of packed
@@ -143,6 +154,20 @@ b40c_status b40c_graph_read(const char* path, int32_t format, int32_t undirected
  * neighbours per vertex (WriteMetisStream, metis.cuh:220-245); path NULL:
  * stdout. */
 b40c_status b40c_graph_write_metis(const b40c_graph* g, const char* path);
+
+/* ---- consecutive removal and scan ------------------------------------- */
+/* consecutive_removal::Enactor::Trim (b40c/consecutive_removal/enactor.cuh:
+ * 119-126) on device arrays: keeps the first key (and value) of every run of
+ * equal consecutive keys, in order; *num_compacted receives the count.  Values
+ * are optional (both NULL or both set); outputs hold n entries and must not
+ * alias the inputs. */
+b40c_status b40c_consecutive_removal_u64(const uint64_t* d_keys_in, const uint32_t* d_values_in,
+                                         uint64_t* d_keys_out, uint32_t* d_values_out, int64_t n,
+                                         int64_t* num_compacted);
+/* scan::Enactor::Scan (b40c/scan/enactor.cuh:122-136): prefix sum of n
+ * unsigned 32-bit values (mod 2^32), exclusive or inclusive; d_out may equal
+ * d_in; *total (nullable) receives the exact 64-bit sum. */
+b40c_status b40c_scan_u32(const uint32_t* d_in, uint32_t* d_out, int64_t n, int32_t exclusive, uint64_t* total);
 
 /* ---- radix sort (b40c/radix_sort/enactor.cuh:153-266) ------------------- */
 /* Stable LSD sort of device keys over bits [begin_bit, end_bit) with 8-bit

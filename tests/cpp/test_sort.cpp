@@ -2,6 +2,9 @@
 // test/radix_sort/test_sort.cu (:185-261): random keys (and values), sort on
 // the device through the C++ facade (include/b40c/radix_sort.hpp), compare
 // with std::sort of the keys / std::stable_sort of (key, value) pairs by key.
+// Also the consecutive removal (test/consecutive_removal/
+// test_consecutive_removal.cu:135-159: first of each run) and the exclusive
+// scan through include/b40c/primitives.hpp.
 //
 // usage: test_sort [n] [seed]    exit status 0 iff every case matches.
 #include <algorithm>
@@ -11,6 +14,7 @@
 #include <numeric>
 #include <vector>
 
+#include "b40c/primitives.hpp"
 #include "b40c/radix_sort.hpp"
 
 namespace {
@@ -80,6 +84,60 @@ bool pairs_case(size_t n, uint64_t seed) {
     return ok;
 }
 
+// Sorted keys with runs, trimmed with values; reference = first of each run.
+bool trim_case(size_t n, uint64_t seed) {
+    std::vector<uint64_t> k(n);
+    std::vector<uint32_t> v(n);
+    for (size_t i = 0; i < n; ++i) {
+        k[i] = splitmix(seed) % (n / 3 + 1);
+        v[i] = (uint32_t)i;
+    }
+    std::sort(k.begin(), k.end());
+    uint64_t* dk0 = device_copy(k);
+    uint64_t* dk1 = device_copy(std::vector<uint64_t>(n));
+    uint32_t* dv0 = device_copy(v);
+    uint32_t* dv1 = device_copy(std::vector<uint32_t>(n));
+    b40c::util::DoubleBuffer<uint64_t, uint32_t> buf(dk0, dk1, dv0, dv1);
+    b40c::consecutive_removal::Enactor trimmer;
+    int64_t m = -1;
+    const b40c_status s = trimmer.Trim(buf, (int64_t)n, &m);
+    std::vector<uint64_t> gk = host_copy(buf.d_keys[buf.selector], m > 0 ? (size_t)m : 0);
+    std::vector<uint32_t> gv = host_copy(buf.d_values[buf.selector], m > 0 ? (size_t)m : 0);
+    for (void* p : {(void*)dk0, (void*)dk1, (void*)dv0, (void*)dv1}) b40c_device_free(p);
+    std::vector<uint64_t> ek;
+    std::vector<uint32_t> ev;
+    for (size_t i = 0; i < n; ++i)
+        if (i == 0 || k[i] != k[i - 1]) {
+            ek.push_back(k[i]);
+            ev.push_back(v[i]);
+        }
+    const bool ok = s == B40C_OK && m == (int64_t)ek.size() && gk == ek && gv == ev;
+    printf("trim n=%zu -> %lld: %s\n", n, (long long)m, ok ? "CORRECT" : "INCORRECT");
+    return ok;
+}
+
+bool scan_case(size_t n, uint64_t seed) {
+    std::vector<uint32_t> x(n);
+    for (auto& e : x) e = (uint32_t)splitmix(seed);
+    uint32_t* d = device_copy(x);
+    b40c::scan::Enactor scanner;
+    uint64_t total = 0;
+    const b40c_status s = scanner.Scan<true>(d, d, (int64_t)n, &total);
+    std::vector<uint32_t> got = host_copy(d, n);
+    b40c_device_free(d);
+    std::vector<uint32_t> ref(n);
+    uint32_t acc = 0;
+    uint64_t exact = 0;
+    for (size_t i = 0; i < n; ++i) {
+        ref[i] = acc;
+        acc += x[i];
+        exact += x[i];
+    }
+    const bool ok = s == B40C_OK && got == ref && total == exact;
+    printf("exclusive scan n=%zu: %s\n", n, ok ? "CORRECT" : "INCORRECT");
+    return ok;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -91,6 +149,10 @@ int main(int argc, char** argv) {
     ok &= pairs_case(n, seed + 2);
     ok &= keys_case(1, seed + 3, false);
     ok &= pairs_case(0, seed + 4);
+    ok &= trim_case(n, seed + 5);
+    ok &= trim_case(1, seed + 6);
+    ok &= scan_case(n, seed + 7);
+    ok &= scan_case(0, seed + 8);
     printf("summary: %s\n", ok ? "all CORRECT" : "FAILURES");
     return ok ? 0 : 1;
 }
