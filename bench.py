@@ -368,6 +368,57 @@ def bench_ingest(scale=20, text_scale=16):
     return out
 
 
+def bench_primitives(hip, n_scan=1 << 28, n_trim=1 << 27, reps=5):
+    """Consecutive removal and prefix scan at HBM-filling sizes, timed with
+    HIP events on the stream they run on; algorithmic bytes: scan 4 B read +
+    4 B written per element, trim 8 B read per key + 8 B written per kept
+    key.  Results are checked against numpy on the host."""
+    from back40computing_amd import _lib, primitives
+    from back40computing_amd.radix_sort import DeviceArray, DoubleBuffer
+    out = {}
+    rng = np.random.default_rng(1)
+    x = rng.integers(0, 1 << 20, n_scan).astype(np.uint32)
+    d = DeviceArray(x.nbytes)
+    d.upload(x)
+    e0, e1 = hip.event(), hip.event()
+    best = None
+    for _ in range(reps):
+        d.upload(x)
+        hip.sync()
+        hip.record(e0, None)
+        primitives.Scan().Scan(d, d, n_scan, True)
+        hip.record(e1, None)
+        ms = hip.elapsed_ms(e0, e1)
+        best = ms if best is None else min(best, ms)
+    got = d.download(np.uint32, n_scan)
+    ref = (np.cumsum(x, dtype=np.uint64) - x).astype(np.uint32)
+    gbs = 8.0 * n_scan / (best * 1e-3) / 1e9
+    out["scan_u32"] = {"n": n_scan, "ms": round(best, 4), "gb_per_s": round(gbs, 1),
+                       "roofline_frac": round(gbs / HBM_PEAK_GBS, 4), "correct": bool(np.array_equal(got, ref))}
+    d.free()
+    keys = np.cumsum(rng.integers(0, 2, n_trim)).astype(np.uint64)  # non-decreasing, runs of equal keys
+    buf = DoubleBuffer(np.uint64, n_trim)
+    best, m = None, 0
+    for _ in range(reps):
+        buf.selector = 0
+        buf.d_keys[0].upload(keys)
+        hip.sync()
+        hip.record(e0, None)
+        m = primitives.ConsecutiveRemoval().Trim(buf, n_trim)
+        hip.record(e1, None)
+        ms = hip.elapsed_ms(e0, e1)
+        best = ms if best is None else min(best, ms)
+    head = np.ones(n_trim, bool)
+    head[1:] = keys[1:] != keys[:-1]
+    ok = m == int(head.sum()) and np.array_equal(buf.d_keys[buf.selector].download(np.uint64, m), keys[head])
+    gbs = (8.0 * n_trim + 8.0 * m) / (best * 1e-3) / 1e9
+    out["trim_u64"] = {"n": n_trim, "kept": m, "ms": round(best, 4), "gb_per_s": round(gbs, 1),
+                       "roofline_frac": round(gbs / HBM_PEAK_GBS, 4), "correct": bool(ok)}
+    for a in buf.d_keys:
+        a.free()
+    return out
+
+
 def pick_sources_cached(off, seed, count):
     from back40computing_amd.sources import pick_sources
     return pick_sources(off, seed, count)
@@ -542,6 +593,14 @@ def main():
         except b4.B40CError as ex:
             lattice = {"error": str(ex)}
             log("lattice workload failed:", ex)
+    prims = None
+    if not args.no_ingest:
+        try:
+            prims = bench_primitives(hip)
+            log("primitives", json.dumps(prims))
+        except b4.B40CError as ex:
+            prims = {"error": str(ex)}
+            log("primitives workload failed:", ex)
     ingest = None
     if not args.no_ingest:
         try:
@@ -606,6 +665,7 @@ def main():
         "batched": batched,
         "served": served,
         "ingest": ingest,
+        "primitives": prims,
         "detail": {"gteps_enact_only": round(head["gteps_enact_only"], 3),
                    "ms_per_search_enact": round(head["ms_per_search_enact"], 4),
                    "b_alg_per_search": head["b_alg_per_search"],

@@ -135,3 +135,33 @@ def test_bad_arguments(prim):
     from back40computing_amd import bfs
     with pytest.raises(B40CError):
         bfs.Graph.from_coo_unique(4, [0, 5], [1, 0])
+
+
+def test_unaligned_pointers_use_the_scalar_path(prim):
+    """Device pointers not on a 16-byte boundary take the element-wise
+    kernels; results must not change."""
+    import ctypes
+    from back40computing_amd import _lib
+    from back40computing_amd.radix_sort import DeviceArray
+    L = _lib.lib()
+    rng = np.random.default_rng(8)
+    n = 300_001
+    x = rng.integers(0, 1000, n).astype(np.uint32)
+    d = DeviceArray((n + 4) * 4)
+    d.upload(np.concatenate([np.zeros(1, np.uint32), x]))
+    off = ctypes.c_void_p(d.ptr.value + 4)
+    tot = ctypes.c_uint64()
+    _lib.check(L.b40c_scan_u32(off, off, n, 1, ctypes.byref(tot)))
+    got = d.download(np.uint32, n + 1)[1:]
+    np.testing.assert_array_equal(got, (np.cumsum(x, dtype=np.uint64) - x).astype(np.uint32))
+    k = np.cumsum(rng.integers(0, 2, n)).astype(np.uint64)
+    kin = DeviceArray((n + 1) * 8)
+    kout = DeviceArray((n + 1) * 8)
+    kin.upload(np.concatenate([np.zeros(1, np.uint64), k]))
+    cnt = ctypes.c_int64()
+    _lib.check(L.b40c_consecutive_removal_u64(ctypes.c_void_p(kin.ptr.value + 8), None,
+                                              ctypes.c_void_p(kout.ptr.value + 8), None, n, ctypes.byref(cnt)))
+    head = np.ones(n, bool)
+    head[1:] = k[1:] != k[:-1]
+    assert cnt.value == int(head.sum())
+    np.testing.assert_array_equal(kout.download(np.uint64, n + 1)[1:cnt.value + 1], k[head])
