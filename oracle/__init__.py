@@ -56,6 +56,9 @@ def lib():
         L.orc_check_pred.restype = c_i64
         L.orc_edges_visited.argtypes = [c_i32, _i32p, _i32p, ctypes.POINTER(c_i64)]
         L.orc_edges_visited.restype = c_i64
+        _i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+        L.orc_histogram.argtypes = [c_i32, _i32p, _i32p, _i32p, c_i32, _i64p, _i64p, _i64p]
+        L.orc_histogram.restype = c_i32
         L.orc_pick_sources.argtypes = [c_i32, _i32p, c_u64, c_i32, _i32p]
         L.orc_pick_sources.restype = c_i32
         L.orc_sort_u64.argtypes = [_u64p, c_i64]
@@ -162,6 +165,20 @@ def edges_visited(offsets, labels):
     e = lib().orc_edges_visited(offsets.size - 1, offsets, np.ascontiguousarray(labels, np.int32),
                                 ctypes.byref(nv))
     return int(e), int(nv.value)
+
+
+def histogram(offsets, cols, labels, search_depth):
+    """(expanded, unique_expanded, discovered) per depth, each of length
+    search_depth + 1 (test_bfs.cu:246-337)."""
+    ex = np.zeros(search_depth + 1, np.int64)
+    ue = np.zeros(search_depth + 1, np.int64)
+    dc = np.zeros(search_depth + 1, np.int64)
+    rc = lib().orc_histogram(offsets.size - 1, np.ascontiguousarray(offsets, np.int32),
+                             np.ascontiguousarray(cols, np.int32), np.ascontiguousarray(labels, np.int32),
+                             int(search_depth), ex, ue, dc)
+    if rc != 0:
+        raise ValueError("label outside [-1, search_depth)")
+    return ex, ue, dc
 
 
 def pick_sources(offsets, seed, count):

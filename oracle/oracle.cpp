@@ -339,6 +339,35 @@ int64_t orc_edges_visited(int32_t n, const int32_t* offsets, const int32_t* labe
     return e;
 }
 
+// Work histogram by depth (Histogram, test_bfs.cu:246-337), from a
+// search's depth labels: discovered[d] = vertices at depth d;
+// expanded[d + 1] = degree sum of the depth-d vertices (the edges a
+// level-synchronous search expands to reach depth d + 1); unique_expanded
+// [d + 1] = distinct endpoints of those edges; level 0 expands the source
+// alone (expanded = unique = 1).  Arrays hold search_depth + 1 entries.
+// Returns 0, or -1 for a label outside [-1, search_depth).
+int32_t orc_histogram(int32_t n, const int32_t* offsets, const int32_t* cols, const int32_t* labels,
+                      int32_t search_depth, int64_t* expanded, int64_t* unique_expanded, int64_t* discovered) {
+    const int32_t D = search_depth;
+    for (int32_t d = 0; d <= D; ++d) expanded[d] = unique_expanded[d] = discovered[d] = 0;
+    if (D < 1) return 0;
+    expanded[0] = unique_expanded[0] = 1;
+    std::vector<std::vector<int32_t>> frontier(D);
+    for (int32_t v = 0; v < n; ++v) {
+        const int32_t d = labels[v];
+        if (d < -1 || d >= D) return -1;
+        if (d < 0) continue;
+        ++discovered[d];
+        expanded[d + 1] += offsets[v + 1] - offsets[v];
+        frontier[d].insert(frontier[d].end(), cols + offsets[v], cols + offsets[v + 1]);
+    }
+    for (int32_t d = 0; d < D; ++d) {
+        std::sort(frontier[d].begin(), frontier[d].end());
+        unique_expanded[d + 1] = std::unique(frontier[d].begin(), frontier[d].end()) - frontier[d].begin();
+    }
+    return 0;
+}
+
 // Seeded source choice among vertices of nonzero degree: candidate k is
 // splitmix64(splitmix64(seed ^ 0x5EED) + k) mod n, accepted when its degree
 // is positive and it was not chosen before (the reference re-rolls sources

@@ -11,10 +11,16 @@
 //                 [--undirected] [--splice=<n>]
 //                 [--strategy=0..5] [--mode=default|fused|iterative]
 //                 [--i=<sources>] [--src=<vertex>] [--seed=<n>]
-//                 [--queue-sizing=<f>]
+//                 [--queue-sizing=<f>] [--instrumented] [--v2]
 // Prints one line per source and a summary; exit status 0 iff every source
-// is CORRECT.
+// is CORRECT.  As in the reference: --instrumented runs the instrumented
+// enactors (average workgroup duty), each source's line carries total queued
+// and redundant work ((queued - edges visited) / edges visited,
+// test_bfs.cu:369-373), the summary the mean, deviation and cv of search
+// depth, redundant work, duty and rate (test_bfs.cu:451-478), and --v2 prints
+// the work histogram by depth of every source (test_bfs.cu:246-337, 560).
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -31,6 +37,8 @@ int64_t orc_check_pred(int32_t n, const int32_t* offsets, const int32_t* cols, c
                        const int32_t* pred);
 int64_t orc_edges_visited(int32_t n, const int32_t* offsets, const int32_t* labels, int64_t* nodes_visited);
 int32_t orc_pick_sources(int32_t n, const int32_t* offsets, uint64_t seed, int32_t count, int32_t* out);
+int32_t orc_histogram(int32_t n, const int32_t* offsets, const int32_t* cols, const int32_t* labels,
+                      int32_t search_depth, int64_t* expanded, int64_t* unique_expanded, int64_t* discovered);
 }
 
 using namespace b40c::graph::bfs;
@@ -47,6 +55,8 @@ struct Options {
     int src = -1;
     unsigned long long seed = 1;
     double queue_sizing = 1.3;
+    bool instrumented = false;
+    bool v2 = false;
     std::string file;
     bool undirected = false;  // file graphs: add a back edge per edge (test_bfs.cu:182-185)
     int splice = 0;
@@ -65,6 +75,8 @@ bool parse(int argc, char** argv, Options& o) {
         else if (!strncmp(a, "--seed=", 7)) o.seed = strtoull(a + 7, nullptr, 10);
         else if (!strncmp(a, "--queue-sizing=", 15)) o.queue_sizing = atof(a + 15);
         else if (!strcmp(a, "--undirected")) o.undirected = true;
+        else if (!strcmp(a, "--instrumented")) o.instrumented = true;
+        else if (!strcmp(a, "--v2")) o.v2 = true;
         else if (!strncmp(a, "--splice=", 9)) o.splice = atoi(a + 9);
         else if (a[0] == '-') return false;
         else if (pos == 0) { o.kind = a; ++pos; }
@@ -75,6 +87,33 @@ bool parse(int argc, char** argv, Options& o) {
         else return false;
     }
     return o.kind == "rmat" || o.kind == "grid3d" || (file_kind(o.kind) && !o.file.empty());
+}
+
+// Running mean and sample variance (Welford).
+struct RunningStat {
+    long long count = 0;
+    double mean = 0.0, m2 = 0.0;
+    void add(double x) {
+        ++count;
+        const double d = x - mean;
+        mean += d / count;
+        m2 += d * (x - mean);
+    }
+    double stddev() const { return count > 1 ? std::sqrt(m2 / (count - 1)) : 0.0; }
+    void print(const char* name) const {
+        printf("    [%s] u: %.3f, s: %.3f, cv: %.4f\n", name, mean, stddev(), mean != 0.0 ? stddev() / mean : 0.0);
+    }
+};
+
+void print_histogram(const std::vector<int32_t>& offs, const std::vector<int32_t>& cols,
+                     const std::vector<int32_t>& ref, int32_t search_depth) {
+    const int32_t n = (int32_t)offs.size() - 1;
+    std::vector<int64_t> ex(search_depth + 1), ue(search_depth + 1), dc(search_depth + 1);
+    if (orc_histogram(n, offs.data(), cols.data(), ref.data(), search_depth, ex.data(), ue.data(), dc.data()) != 0)
+        return;
+    printf("Work Histogram:\nDepth, Expanded, Unique-Expanded, Discovered\n");
+    for (int32_t d = 0; d < search_depth; ++d)
+        printf("%d, %lld, %lld, %lld\n", d, (long long)ex[d], (long long)ue[d], (long long)dc[d]);
 }
 
 // The enactor classes are distinct types (as in the reference); run the
@@ -104,6 +143,7 @@ int run(const Options& o, b40c_graph* g, const std::vector<int32_t>& offs, const
     int failures = 0;
     double total_ms = 0.0;
     long long total_edges = 0;
+    RunningStat st_depth, st_redundant, st_duty, st_rate;
     for (int32_t src : sources) {
         s = problem.Reset(enactor.GetFrontierType(), o.queue_sizing);
         if (s != B40C_OK) {
@@ -137,17 +177,49 @@ int run(const Options& o, b40c_graph* g, const std::vector<int32_t>& offs, const
         const int64_t edges_visited = orc_edges_visited(n, offs.data(), ref.data(), &nodes_visited);
         total_ms += ms;
         total_edges += edges_visited;
+        const double rate = ms > 0 ? edges_visited / (ms * 1000.0) : 0.0;
+        const double redundant =
+            (queued > 0 && edges_visited > 0) ? 100.0 * ((double)queued - edges_visited) / edges_visited : 0.0;
         printf("src %d: %s depth=%s pred=%s search_depth=%d (ref %d) nodes_visited=%lld edges_visited=%lld "
-               "elapsed=%.3f ms rate=%.2f MiEdges/s\n",
+               "total_queued=%lld redundant_work=%.2f%% elapsed=%.3f ms rate=%.2f MiEdges/s",
                src, ok ? "CORRECT" : "INCORRECT", depth_ok ? "ok" : "MISMATCH", bad_pred < 0 ? "ok" : "BAD",
-               search_depth, ref_sd, (long long)nodes_visited, (long long)edges_visited, ms,
-               ms > 0 ? edges_visited / (ms * 1000.0) : 0.0);
+               search_depth, ref_sd, (long long)nodes_visited, (long long)edges_visited, queued, redundant, ms,
+               rate);
+        if (duty > 0) printf(" avg_duty=%.2f%%", duty * 100.0);
+        printf("\n");
+        if (o.v2) print_histogram(offs, cols, ref, ref_sd);
+        st_depth.add(search_depth);
+        st_redundant.add(redundant);
+        st_duty.add(duty * 100.0);
+        st_rate.add(rate);
         failures += ok ? 0 : 1;
     }
-    printf("summary: strategy=%d mode=%s sources=%zu failures=%d avg_elapsed=%.3f ms avg_rate=%.2f MiEdges/s\n",
-           o.strategy, o.mode.c_str(), sources.size(), failures, sources.empty() ? 0.0 : total_ms / sources.size(),
-           total_ms > 0 ? total_edges / (total_ms * 1000.0) : 0.0);
+    if (st_rate.count > 0) {
+        printf("  Summary after %lld test iterations (bias-corrected):\n", st_rate.count);
+        st_depth.print("Search depth");
+        st_redundant.print("redundant work %");
+        if (st_duty.mean > 0) st_duty.print("Duty %");
+        st_rate.print("Rate MiEdges/s");
+    }
+    printf("summary: strategy=%d mode=%s instrumented=%d sources=%zu failures=%d avg_elapsed=%.3f ms "
+           "avg_rate=%.2f MiEdges/s\n",
+           o.strategy, o.mode.c_str(), o.instrumented ? 1 : 0, sources.size(), failures,
+           sources.empty() ? 0.0 : total_ms / sources.size(), total_ms > 0 ? total_edges / (total_ms * 1000.0) : 0.0);
     return failures ? 1 : 0;
+}
+
+template <bool INSTRUMENT>
+int run_strategy(const Options& o, b40c_graph* g, const std::vector<int32_t>& offs, const std::vector<int32_t>& cols) {
+    switch (o.strategy) {
+        case B40C_EXPAND_CONTRACT: return run<EnactorExpandContract<INSTRUMENT>>(o, g, offs, cols);
+        case B40C_CONTRACT_EXPAND: return run<EnactorContractExpand<INSTRUMENT>>(o, g, offs, cols);
+        case B40C_TWO_PHASE: return run<EnactorTwoPhase<INSTRUMENT>>(o, g, offs, cols);
+        case B40C_HYBRID: return run<EnactorHybrid<INSTRUMENT>>(o, g, offs, cols);
+        case B40C_PERSISTENT: return run<EnactorPersistent<INSTRUMENT>>(o, g, offs, cols);
+        case B40C_DIRECTION_OPTIMIZING: return run<EnactorDirectionOptimizing<INSTRUMENT>>(o, g, offs, cols);
+    }
+    fprintf(stderr, "unknown strategy %d\n", o.strategy);
+    return 2;
 }
 
 }  // namespace
@@ -157,7 +229,8 @@ int main(int argc, char** argv) {
     if (!parse(argc, argv, o)) {
         fprintf(stderr, "usage: %s rmat <scale> [edge_factor] | grid3d <width> | market <file> | metis <file> "
                         "| dimacs <file> [--undirected] [--splice=N] [--strategy=0..5] "
-                        "[--mode=default|fused|iterative] [--i=N] [--src=V] [--seed=S] [--queue-sizing=F]\n",
+                        "[--mode=default|fused|iterative] [--i=N] [--src=V] [--seed=S] [--queue-sizing=F] "
+                        "[--instrumented] [--v2]\n",
                 argv[0]);
         return 2;
     }
@@ -180,15 +253,7 @@ int main(int argc, char** argv) {
     b40c_graph_copy_to_host(g, offs.data(), cols.data());
     printf("graph %s: nodes=%d edges=%lld\n", o.kind.c_str(), n, (long long)m);
     int rc = 2;
-    switch (o.strategy) {
-        case B40C_EXPAND_CONTRACT: rc = run<EnactorExpandContract<false>>(o, g, offs, cols); break;
-        case B40C_CONTRACT_EXPAND: rc = run<EnactorContractExpand<false>>(o, g, offs, cols); break;
-        case B40C_TWO_PHASE: rc = run<EnactorTwoPhase<false>>(o, g, offs, cols); break;
-        case B40C_HYBRID: rc = run<EnactorHybrid<false>>(o, g, offs, cols); break;
-        case B40C_PERSISTENT: rc = run<EnactorPersistent<false>>(o, g, offs, cols); break;
-        case B40C_DIRECTION_OPTIMIZING: rc = run<EnactorDirectionOptimizing<false>>(o, g, offs, cols); break;
-        default: fprintf(stderr, "unknown strategy %d\n", o.strategy);
-    }
+    rc = o.instrumented ? run_strategy<true>(o, g, offs, cols) : run_strategy<false>(o, g, offs, cols);
     b40c_graph_destroy(g);
     return rc;
 }

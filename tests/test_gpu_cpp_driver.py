@@ -92,3 +92,34 @@ def test_cpp_driver_file_graphs(tmp_path, kind):
         assert r.returncode == 0, r.stdout + r.stderr
         assert "graph %s: nodes=%d edges=%d" % (kind, n, 2 * len(edges)) in r.stdout
         assert "failures=0" in r.stdout
+
+
+@pytest.mark.parametrize("strategy", ["--strategy=0", "--strategy=2", "--strategy=3"])
+def test_cpp_driver_instrumented_histogram(strategy):
+    """--instrumented runs the instrumented enactors (average workgroup duty
+    reported), every source line carries total queued and redundant work, the
+    summary the bias-corrected statistics, and --v2 the work histogram by
+    depth (test_bfs.cu:246-337, 343-490)."""
+    if not os.path.exists(DRIVER):
+        pytest.fail("C++ driver not built (run __graft_entry__.build())")
+    print("driver instrumented case:", strategy, flush=True)
+    r = subprocess.run([DRIVER, "rmat", "12", strategy, "--i=3", "--seed=4", "--instrumented", "--v2"],
+                       capture_output=True, text=True, timeout=120)
+    print(r.stdout[-2500:])
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "failures=0" in r.stdout and "instrumented=1" in r.stdout
+    assert r.stdout.count("Work Histogram:") == 3
+    assert "avg_duty=" in r.stdout and "redundant_work=" in r.stdout
+    assert "Summary after 3 test iterations" in r.stdout and "[Duty %]" in r.stdout
+    # The histogram's discovered column sums to the nodes visited of its source.
+    lines = r.stdout.splitlines()
+    i = lines.index("Depth, Expanded, Unique-Expanded, Discovered")
+    src_line = lines[i - 2]
+    nv = int(src_line.split("nodes_visited=")[1].split()[0])
+    disc = 0
+    for ln in lines[i + 1:]:
+        parts = ln.split(", ")
+        if len(parts) != 4:
+            break
+        disc += int(parts[3])
+    assert disc == nv

@@ -161,3 +161,33 @@ def test_edges_visited_and_sources(orc):
     mask = depth >= 0
     assert nv == int(mask.sum())
     assert e == int(np.diff(off)[mask].sum())
+
+
+def test_work_histogram_hand_graph(orc):
+    """Histogram (test_bfs.cu:246-337) on a path 0-1-2-3 with leaves 4, 5 on
+    vertex 1, from source 0, worked by hand: depths 0 | 1 | 2 4 5 | 3."""
+    adj = {0: [1], 1: [0, 2, 4, 5], 2: [1, 3], 3: [2], 4: [1], 5: [1]}
+    off = np.cumsum([0] + [len(adj[v]) for v in range(6)]).astype(np.int32)
+    cols = np.array([x for v in range(6) for x in adj[v]], np.int32)
+    d, _, sd = orc.bfs(off, cols, 0, with_pred=False)
+    assert sd == 4
+    ex, ue, dc = orc.histogram(off, cols, d, sd)
+    assert dc.tolist() == [1, 1, 3, 1, 0]
+    assert ex.tolist() == [1, 1, 4, 4, 1]   # degree sums of the previous depth
+    assert ue.tolist() == [1, 1, 4, 2, 1]   # {1}, {0,2,4,5}, {1,3}, {2}
+
+
+def test_work_histogram_invariants(orc):
+    """On an RMAT graph: discovered sums to the vertices reached, expanded
+    (after level 0) to the edges visited, and every vertex discovered at
+    depth d+1 is among the unique endpoints expanded from depth d."""
+    off, cols = orc.build_rmat_csr(12, 8, seed=3)
+    src = int(orc.pick_sources(off, 4, 1)[0])
+    d, _, sd = orc.bfs(off, cols, src, with_pred=False)
+    ex, ue, dc = orc.histogram(off, cols, d, sd)
+    nv = int((d >= 0).sum())
+    ev, _ = orc.edges_visited(off, d)
+    assert dc.sum() == nv
+    assert ex[1:].sum() == ev
+    assert np.all(ue[1:] <= ex[1:])
+    assert np.all(dc[1:] <= ue[1:])
