@@ -33,7 +33,7 @@ SPACE = {
     "grid_gather_threshold": [1, 16, 32, 64, 128, 256, 512, 1024, 2048, 8192, 65536],
     "cta_gather_threshold": [128, 256, 512, 1024, 4096],
     "wave_gather_threshold": [16, 32, 64, 128],
-    "grid_size": [256, 512, 1024, 2048, 4096],
+    "grid_size": [256, 512, 1024, 1536, 2048, 4096],
     "levels_per_sync": [2, 4, 6, 7, 8, 12, 16, 32, 64],
     "saturation_quit": [64, 256, 512, 2048],
     "bottom_up_grid_size": [512, 1024, 1280, 1536, 2048, 4096],
