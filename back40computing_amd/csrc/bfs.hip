@@ -873,11 +873,18 @@ __global__ __launch_bounds__(BFS_THREADS) void k_fused(BfsArgs a, int ph, int32_
             ce_phase(a, ph, depth, s);
             ph += 1;
         } else {
+            // The key queue and the pair queue share LDS (LevelShared's
+            // union): the key queue's count lies under the pair queue's
+            // values, so each step starts its own queue from zero.
+            if (threadIdx.x == 0) s.qv.count = 0;
+            __syncthreads();
             tp_contract_phase(a, ph, depth, s);
             if (a.wg_stats && threadIdx.x == 0) busy += wg_clock() - tb;
             grid_barrier(&a.ctr->bar_count, &a.ctr->bar_gen, gridDim.x, &a.ctr->bar_abort);
             if (ld_relaxed(&a.ctr->bar_abort)) break;
             if (a.wg_stats && threadIdx.x == 0) tb = wg_clock();
+            if (threadIdx.x == 0) s.qp.count = 0;
+            __syncthreads();
             tp_expand_phase(a, ph + 1, s);
             ph += 2;
         }
