@@ -49,6 +49,34 @@ COO_DROP_SELF_LOOPS = 1
 COO_SYMMETRIZE = 2
 
 
+# b40c_microbench_search modes
+MB_GATHER_LOOKUP = 0
+MB_CULL = 1
+
+
+class MbLevel(ctypes.Structure):
+    """b40c_mb_level: one iteration of the per-stage microbenchmark search."""
+    _fields_ = [
+        ("iteration", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("lookup_in", ctypes.c_int64),
+        ("bitmask_pass", ctypes.c_int64),
+        ("label_pass", ctypes.c_int64),
+        ("local_pass", ctypes.c_int64),
+        ("discovered", ctypes.c_int64),
+        ("gathered", ctypes.c_int64),
+        ("gather_checksum", ctypes.c_uint64),
+        ("offsets_ms", ctypes.c_float),
+        ("bench_lookup_ms", ctypes.c_float),
+        ("lookup_ms", ctypes.c_float),
+        ("bench_gather_ms", ctypes.c_float),
+        ("gather_ms", ctypes.c_float),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
 class B40CError(RuntimeError):
     """A non-OK b40c_status; ``.status`` holds the code."""
 
@@ -128,6 +156,7 @@ SIGNATURES = {
     "b40c_enactor_destroy": (None, [_vp]),
     "b40c_gbarrier_benchmark": (_i32, [_i32, _i32, _p_d]),
     "b40c_gbarrier_check": (_i32, [_i32, _i32, _p_i64]),
+    "b40c_microbench_search": (_i32, [_vp, _i32, _i32, _vp, ctypes.POINTER(MbLevel), _i32, _p_i32]),
     "b40c_multigraph_pack": (_i32, [_vp, _i32, _pp]),
     "b40c_multigraph_info": (_i32, [_vp, _p_i32, _vp, _vp]),
     "b40c_multigraph_destroy": (None, [_vp]),

@@ -44,6 +44,12 @@ b40c_status radix_sort_u64(uint64_t* k0, uint64_t* k1, int64_t n, int begin_bit,
 b40c_status radix_sort_pairs_u32(uint32_t* k0, uint32_t* k1, uint32_t* v0, uint32_t* v1, int64_t n, int begin_bit,
                                  int end_bit, int32_t* selector, hipStream_t stream);
 
+// primitives.hip: prefix scan of uint32 on `stream` with caller-provided
+// scratch (`partials`: scan_u32_max_partials() entries; `total`: the exact sum).
+int scan_u32_max_partials();
+b40c_status scan_u32_async(const uint32_t* in, uint32_t* out, int64_t n, bool exclusive, uint64_t* partials,
+                           uint64_t* total, hipStream_t stream);
+
 }  // namespace b40c
 
 // Device-resident CSR graph (reference counted).

@@ -276,6 +276,35 @@ void chunking(int64_t n, int64_t step, int64_t& chunk, int& G) {
 }
 
 }  // namespace
+
+int scan_u32_max_partials() {
+    DeviceInfo info;
+    const int cus = (device_info(info) == B40C_OK) ? info.num_cus : 256;
+    return cus * 8;
+}
+
+b40c_status scan_u32_async(const uint32_t* in, uint32_t* out, int64_t n, bool exclusive, uint64_t* partials,
+                           uint64_t* total, hipStream_t stream) {
+    if (n <= 0) return B40C_OK;
+    const bool vec = aligned16(in) && aligned16(out);
+    int64_t chunk;
+    int G;
+    chunking(n, vec ? STEP32 : PT, chunk, G);
+    if (vec)
+        hipLaunchKernelGGL(k_scan_reduce_v, dim3(G), dim3(PT), 0, stream, in, n, chunk, partials);
+    else
+        hipLaunchKernelGGL(k_scan_reduce, dim3(G), dim3(PT), 0, stream, in, n, chunk, partials);
+    hipLaunchKernelGGL(k_spine_u64, dim3(1), dim3(PT), 0, stream, partials, G, total);
+    if (vec)
+        hipLaunchKernelGGL(k_scan_down_v, dim3(G), dim3(PT), 0, stream, in, out, n, chunk, partials,
+                           exclusive ? 0 : 1);
+    else
+        hipLaunchKernelGGL(k_scan_down, dim3(G), dim3(PT), 0, stream, in, out, n, chunk, partials,
+                           exclusive ? 0 : 1);
+    B40C_HIP(hipGetLastError());
+    return B40C_OK;
+}
+
 }  // namespace b40c
 
 using b40c::fail;
@@ -323,25 +352,10 @@ b40c_status b40c_scan_u32(const uint32_t* d_in, uint32_t* d_out, int64_t n, int3
         return fail(B40C_ERR_INVALID_ARG, "b40c_scan_u32: bad arguments");
     if (total) *total = 0;
     if (n == 0) return B40C_OK;
-    const bool vec = aligned16(d_in) && aligned16(d_out);
-    int64_t chunk;
-    int G;
-    chunking(n, vec ? STEP32 : PT, chunk, G);
     DeviceBuffer sums, tot;
-    B40C_TRY(sums.alloc((size_t)G * 8));
+    B40C_TRY(sums.alloc((size_t)scan_u32_max_partials() * 8));
     B40C_TRY(tot.alloc(8));
-    if (vec)
-        hipLaunchKernelGGL(k_scan_reduce_v, dim3(G), dim3(PT), 0, nullptr, d_in, n, chunk, sums.as<uint64_t>());
-    else
-        hipLaunchKernelGGL(k_scan_reduce, dim3(G), dim3(PT), 0, nullptr, d_in, n, chunk, sums.as<uint64_t>());
-    hipLaunchKernelGGL(k_spine_u64, dim3(1), dim3(PT), 0, nullptr, sums.as<uint64_t>(), G, tot.as<uint64_t>());
-    if (vec)
-        hipLaunchKernelGGL(k_scan_down_v, dim3(G), dim3(PT), 0, nullptr, d_in, d_out, n, chunk, sums.as<uint64_t>(),
-                           exclusive ? 0 : 1);
-    else
-        hipLaunchKernelGGL(k_scan_down, dim3(G), dim3(PT), 0, nullptr, d_in, d_out, n, chunk, sums.as<uint64_t>(),
-                           exclusive ? 0 : 1);
-    B40C_HIP(hipGetLastError());
+    B40C_TRY(scan_u32_async(d_in, d_out, n, exclusive != 0, sums.as<uint64_t>(), tot.as<uint64_t>(), nullptr));
     uint64_t h = 0;
     B40C_HIP(hipMemcpy(&h, tot.ptr, 8, hipMemcpyDeviceToHost));
     if (total) *total = h;

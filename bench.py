@@ -368,6 +368,33 @@ def bench_ingest(scale=20, text_scale=16):
     return out
 
 
+def bench_microbench(bfs, graph, srcs, strategy, tuning, nsrc=4):
+    """Per-stage microbenchmark search (status lookup and neighbour gather,
+    each as a bench pass and a real pass, HIP events per launch) on the
+    headline graph from its first sources, both modes; the second search of
+    each source is the one counted.  Stage totals carry algorithmic bytes
+    (back40computing_amd.microbench) and GB/s.  The labels are compared with
+    the traversal enactor's."""
+    from back40computing_amd import microbench
+    out = {}
+    for mode in ("gather_lookup", "cull"):
+        levels_all = []
+        match = True
+        for s in srcs[:nsrc]:
+            microbench.search(graph, int(s), mode, want_depth=False)
+            depth, levels = microbench.search(graph, int(s), mode)
+            ref, _, _ = bfs.bfs(graph, int(s), strategy=strategy, tuning=tuning)
+            match = match and bool(np.array_equal(depth, ref))
+            levels_all += levels
+        summ = microbench.summarize(levels_all)
+        out[mode] = {k: summ[k] for k in ("lookup", "bench_lookup", "offsets", "gather", "bench_gather")}
+        out[mode].update({k: summ[k] for k in summ if k.endswith("_frac")})
+        out[mode]["edges_gathered"] = summ["edges_gathered"]
+        out[mode]["labels_match_enactor"] = match
+    out["sources"] = len(srcs[:nsrc])
+    return out
+
+
 def bench_primitives(hip, n_scan=1 << 28, n_trim=1 << 27, reps=5):
     """Consecutive removal and prefix scan at HBM-filling sizes, timed with
     HIP events on the stream they run on; algorithmic bytes: scan 4 B read +
@@ -511,7 +538,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true", help="skip the batched 1,024-query workload")
     ap.add_argument("--no-served", action="store_true", help="skip the served Poisson workload")
     ap.add_argument("--no-lattice", action="store_true", help="skip the 200^3 lattice (high-diameter) workload")
-    ap.add_argument("--no-ingest", action="store_true", help="skip the graph-file read/build workload")
+    ap.add_argument("--no-ingest", action="store_true",
+                    help="skip the graph-file read/build, primitives and per-stage microbenchmark workloads")
     ap.add_argument("--only", choices=["batched", "served"], help=argparse.SUPPRESS)
     ap.add_argument("--probe", action="store_true",
                     help="counter-collection run: a 1 GiB copy plus --steps searches, no JSON line")
@@ -593,6 +621,14 @@ def main():
         except b4.B40CError as ex:
             lattice = {"error": str(ex)}
             log("lattice workload failed:", ex)
+    micro = None
+    if not args.no_ingest:
+        try:
+            micro = bench_microbench(bfs, graph, srcs, args.strategy, head_tuning)
+            log("microbench", json.dumps(micro))
+        except b4.B40CError as ex:
+            micro = {"error": str(ex)}
+            log("microbench workload failed:", ex)
     prims = None
     if not args.no_ingest:
         try:
@@ -666,6 +702,7 @@ def main():
         "served": served,
         "ingest": ingest,
         "primitives": prims,
+        "microbench": micro,
         "detail": {"gteps_enact_only": round(head["gteps_enact_only"], 3),
                    "ms_per_search_enact": round(head["ms_per_search_enact"], 4),
                    "b_alg_per_search": head["b_alg_per_search"],

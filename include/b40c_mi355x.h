@@ -278,6 +278,38 @@ b40c_status b40c_gbarrier_benchmark(int32_t grid_size, int32_t iterations, doubl
  * workgroups' slabs; *mismatches counts stale or wrong words. */
 b40c_status b40c_gbarrier_check(int32_t grid_size, int32_t iterations, int64_t* mismatches);
 
+/* ---- per-stage BFS microbenchmarks ----------------------------------------
+ * A level-synchronous BFS split into its status lookup (edge frontier ->
+ * culled, labelled vertices as row ranges) and neighbour gather (row ranges ->
+ * unculled neighbour ids, load-balanced over equal tiles of the output) stages; each stage runs once as a bench pass (culls
+ * and reads only, on private copies of the visited state) and once for real,
+ * every launch timed with HIP events (EnactorGatherLookup::EnactSearch,
+ * test/bfs/microbench/enactor_gather_lookup.cuh:152-382, and
+ * EnactorCull::EnactSearch, test/bfs/microbench/enactor_cull.cuh:152-385). */
+#define B40C_MB_GATHER_LOOKUP 0 /* bench lookup: bitmask cull + label read (microbench_bfs.cu) */
+#define B40C_MB_CULL 1          /* bench lookup: bitmask, label, wave and history culls (microbench_cull.cu) */
+
+typedef struct b40c_mb_level {
+    int32_t iteration;        /* depth labelled by this level's lookup (0: the source) */
+    int32_t reserved;
+    int64_t lookup_in;        /* ids entering the lookup (the previous gather's output) */
+    int64_t bitmask_pass;     /* bench lookup: ids left after the bitmask cull (-1 at iteration 0) */
+    int64_t label_pass;       /* bench lookup: ... and whose label was unset */
+    int64_t local_pass;       /* B40C_MB_CULL: ... after the wave and history culls; else -1 */
+    int64_t discovered;       /* lookup output: vertices labelled `iteration` */
+    int64_t gathered;         /* gather output: neighbours of those vertices, with repeats */
+    uint64_t gather_checksum; /* bench gather: sum of the gathered neighbour ids (mod 2^64) */
+    float offsets_ms;         /* gather set-up: scan of the row lengths + tile partition */
+    float bench_lookup_ms, lookup_ms, bench_gather_ms, gather_ms;
+} b40c_mb_level;
+
+/* Runs the microbenchmark search from `src`.  Records one b40c_mb_level per
+ * iteration into levels[0 .. max_levels); *num_levels receives the number of
+ * iterations the search took (more than max_levels: the rest were not
+ * recorded).  depth (nullable, host int32[V]) receives the labels. */
+b40c_status b40c_microbench_search(const b40c_graph* g, int32_t src, int32_t mode, int32_t* depth,
+                                   b40c_mb_level* levels, int32_t max_levels, int32_t* num_levels);
+
 #ifdef __cplusplus
 }
 #endif

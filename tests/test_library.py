@@ -62,6 +62,10 @@ def test_host_argument_validation(L):
     assert L.b40c_problem_reset(None, 0, 1.0) == _lib.ERR_INVALID_ARG
     assert L.b40c_enact_search(None, None, 0, 0, 0) == _lib.ERR_INVALID_ARG
     assert L.b40c_enactor_frontier_type(None) == -1
+    # Microbenchmark search without a graph or a level count.
+    n = ctypes.c_int32()
+    assert L.b40c_microbench_search(None, 0, 0, None, None, 0, ctypes.byref(n)) == _lib.ERR_INVALID_ARG
+    assert L.b40c_microbench_search(None, 0, 0, None, None, 0, None) == _lib.ERR_INVALID_ARG
 
 
 def test_strategy_frontier_types_match_reference_numbering():
