@@ -769,9 +769,19 @@ template <int KIND>
 struct HeavySharedOf {
     typedef TdHeavyShared type;
 };
+// Pair grid tier: a batch of the workgroup's chunks is measured first, then
+// reserved in the output queue with one atomic and written in place.
+constexpr int HEAVY_BATCH = 1024;  // chunks per reservation
+struct PairsHeavyShared {
+    int32_t u[HEAVY_BATCH];
+    int32_t start[HEAVY_BATCH];
+    uint32_t off[HEAVY_BATCH + 1];  // exclusive offsets of the chunks' lengths; [n] = total
+    int s_wave[BFS_WAVES];
+    uint32_t base;
+};
 template <>
 struct HeavySharedOf<HK_PAIRS> {
-    typedef LevelShared type;
+    typedef PairsHeavyShared type;
 };
 
 template <int KIND>
@@ -785,53 +795,138 @@ __global__ __launch_bounds__(BFS_THREADS) void k_heavy(BfsArgs a, int ph, int32_
     if (H == 0 || C == 0) return;
     unsigned long long t0 = 0;
     stats_begin(t0, a);
-    if constexpr (KIND == HK_PAIRS) {
-        shared_init(s);
-    } else {
+    if constexpr (KIND != HK_PAIRS) {
         if (threadIdx.x == 0) s.qv.count = 0;
         __syncthreads();
     }
     const GlobalQueue out = out_queue(a, ph);
-    int it = 0;
-    auto chunk = [&](uint32_t c, int32_t& u, int32_t& v, bool& valid) {
-        const int32_t entry = a.chunk_map[c];
-        int32_t start = 0, len = 0;
-        u = -1;
-        if ((uint32_t)entry < H) {
-            const int2 eu = a.heavy[entry];
-            if (valid_vertex(a, eu.x) && (uint32_t)eu.y <= c) {
-                const int32_t s0 = a.offs[eu.x], deg = a.offs[eu.x + 1] - s0;
-                const long long off = (long long)(c - (uint32_t)eu.y) * HEAVY_CHUNK;
+    // HEAVY_ROUND chunks per iteration (grid-strided), each load stage issued
+    // for all of them before the next, so that many dependent chains
+    // (chunk map -> entry -> offsets -> columns) are in flight per workgroup.
+    constexpr int R = HEAVY_ROUND;
+    const uint32_t stride = gridDim.x;
+    auto chunks = [&](uint32_t c0, int32_t (&u)[R], int32_t (&v)[R], bool (&valid)[R]) {
+        int32_t entry[R], start[R], len[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const uint32_t c = c0 + (uint32_t)r * stride;
+            entry[r] = c < C ? a.chunk_map[c] : -1;
+        }
+        int2 eu[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) eu[r] = (uint32_t)entry[r] < H ? a.heavy[entry[r]] : make_int2(-1, 0);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const uint32_t c = c0 + (uint32_t)r * stride;
+            u[r] = -1;
+            start[r] = 0;
+            len[r] = 0;
+            if ((uint32_t)entry[r] < H && valid_vertex(a, eu[r].x) && (uint32_t)eu[r].y <= c) {
+                const int32_t s0 = a.offs[eu[r].x], deg = a.offs[eu[r].x + 1] - s0;
+                const long long off = (long long)(c - (uint32_t)eu[r].y) * HEAVY_CHUNK;
                 if (off < deg) {
-                    u = eu.x;
-                    start = s0 + (int32_t)off;
-                    len = min((int32_t)HEAVY_CHUNK, deg - (int32_t)off);
+                    u[r] = eu[r].x;
+                    start[r] = s0 + (int32_t)off;
+                    len[r] = min((int32_t)HEAVY_CHUNK, deg - (int32_t)off);
                 }
             }
         }
         const int32_t k = (int32_t)threadIdx.x;
-        valid = k < len;
-        v = valid ? a.cols[start + k] : 0;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            valid[r] = k < len[r];
+            v[r] = valid[r] ? a.cols[start[r] + k] : 0;
+        }
     };
     if constexpr (KIND == HK_PAIRS) {
-        PairSink ps{s, out};
-        for (uint32_t c = blockIdx.x; c < C; c += gridDim.x) {
-            int32_t u, v;
-            bool valid;
-            chunk(c, u, v, valid);
-            ps(valid, v, u);
-            if ((++it & 3) == 0) lq_flush(s.qp, out, LQ_CAP / 2);
+        // Every edge of a pair chunk is output, so a chunk's output size is
+        // its length.  Per batch of HEAVY_BATCH of this workgroup's chunks
+        // (c = blockIdx.x + i * stride): each thread measures the chunks
+        // i = 4*tid .. 4*tid+3, a workgroup scan gives every chunk its offset,
+        // one atomic reserves the batch, and the edges are written in place
+        // (consecutive lanes, consecutive slots), HEAVY_ROUND chunks at a time.
+        (void)chunks;
+        const int tid = (int)threadIdx.x;
+        // chunks of this workgroup: c = blockIdx.x + j * stride < C
+        const uint32_t mine = blockIdx.x < C ? (C - blockIdx.x + stride - 1) / stride : 0u;
+        constexpr int PER = HEAVY_BATCH / BFS_THREADS;
+        for (uint32_t b0 = 0; b0 < mine; b0 += HEAVY_BATCH) {
+            const uint32_t nb = min((uint32_t)HEAVY_BATCH, mine - b0);
+            int32_t lens[PER];
+            int32_t mysum = 0;
+#pragma unroll
+            for (int q = 0; q < PER; ++q) {
+                const uint32_t i = (uint32_t)(tid * PER + q);
+                int32_t u = -1, start = 0, len = 0;
+                if (i < nb) {
+                    const uint32_t c = blockIdx.x + (b0 + i) * stride;
+                    const int32_t entry = a.chunk_map[c];
+                    if ((uint32_t)entry < H) {
+                        const int2 eu = a.heavy[entry];
+                        if (valid_vertex(a, eu.x) && (uint32_t)eu.y <= c) {
+                            const int32_t s0 = a.offs[eu.x], deg = a.offs[eu.x + 1] - s0;
+                            const long long off = (long long)(c - (uint32_t)eu.y) * HEAVY_CHUNK;
+                            if (off < deg) {
+                                u = eu.x;
+                                start = s0 + (int32_t)off;
+                                len = min((int32_t)HEAVY_CHUNK, deg - (int32_t)off);
+                            }
+                        }
+                    }
+                    s.u[i] = u;
+                    s.start[i] = start;
+                }
+                lens[q] = len;
+                mysum += len;
+            }
+            int total;
+            int run = wg_exclusive_scan<BFS_THREADS>(mysum, total, s.s_wave);
+#pragma unroll
+            for (int q = 0; q < PER; ++q) {
+                const uint32_t i = (uint32_t)(tid * PER + q);
+                if (i < nb) s.off[i] = (uint32_t)run;
+                run += lens[q];
+            }
+            if (tid == 0) {
+                s.off[nb] = (uint32_t)total;
+                s.base = total ? atomic_add_u32(out.len, (uint32_t)total) : 0u;
+                if (total && (uint64_t)s.base + (uint64_t)total > out.cap) st_relaxed(out.overflow, 1u);
+            }
+            __syncthreads();
+            const uint32_t base = s.base;
+            for (uint32_t i0 = 0; i0 < nb; i0 += R) {
+                int32_t v[R];
+                bool valid[R];
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const uint32_t i = i0 + r;
+                    const int32_t len = i < nb ? (int32_t)(s.off[i + 1] - s.off[i]) : 0;
+                    valid[r] = tid < len;
+                    v[r] = valid[r] ? a.cols[s.start[i] + tid] : 0;
+                }
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    if (!valid[r]) continue;
+                    const uint32_t i = i0 + r;
+                    const uint64_t pos = (uint64_t)base + s.off[i] + (uint32_t)tid;
+                    if (pos < out.cap) {
+                        out.keys[pos] = v[r];
+                        out.vals[pos] = s.u[i];
+                    }
+                }
+            }
+            __syncthreads();
         }
-        lq_flush(s.qp, out, 1);
     } else {
         TdSink tds{a, s.qv, out, (KIND == HK_TD_DO) ? a.fbm[(ph + 1) % 3] : nullptr, depth_next, KIND == HK_TD_DO,
                    false, 0ull};
-        for (uint32_t c = blockIdx.x; c < C; c += gridDim.x) {
-            int32_t u, v;
-            bool valid;
-            chunk(c, u, v, valid);
-            tds(valid, v, u);
-            if ((++it & 3) == 0) lq_flush(s.qv, out, LQ_CAP / 2);
+        for (uint32_t c0 = blockIdx.x; c0 < C; c0 += stride * R) {
+            int32_t u[R], v[R];
+            bool valid[R];
+            chunks(c0, u, v, valid);
+#pragma unroll
+            for (int r = 0; r < R; ++r) tds(valid[r], v[r], u[r]);
+            lq_flush(s.qv, out, LQ_CAP / 2);
         }
         td_finish(a, ph, tds, s.red);
     }

@@ -64,6 +64,7 @@ struct BfsArgs {
 };
 
 constexpr int HEAVY_CHUNK = 256;     // grid-tier edges per work item (one per thread)
+constexpr int HEAVY_ROUND = 4;       // grid-tier work items a workgroup has in flight at once
 
 // ---------------------------------------------------------------------------
 // LDS-buffered queue.
