@@ -279,14 +279,75 @@ __device__ void tp_contract_phase(const BfsArgs& a, int ph, int32_t depth_here, 
     if (__syncthreads_or(found) && threadIdx.x == 0) atomic_max_u32(a.max_depth, (uint32_t)depth_here);
 }
 
+// Positional expansion of one tile of lists (workgroup-uniform call): every
+// edge is output, so the tile's output size is the sum of its degrees.  One
+// atomic reserves it; each position is mapped back to its list by binary
+// search over the tile's ranks (staged in LDS) and written in place as
+// (neighbour, source), EXPAND_ITEMS positions per thread with their loads
+// issued together.  Work per workgroup follows edges, not lists.
+constexpr int EXPAND_ITEMS = 8;
+__device__ __forceinline__ void expand_tile_in_place(const BfsArgs& a, const GlobalQueue& out, int32_t u,
+                                                     int32_t start, int32_t deg, GatherSmem& g) {
+    const int tid = threadIdx.x;
+    int total;
+    const int rank = wg_exclusive_scan<BFS_THREADS>(deg, total, g.s_wave);
+    if (total == 0) return;
+    int32_t* s_rank = g.goff;
+    int32_t* s_start = g.goff + BFS_THREADS;
+    int32_t* s_src = g.gsrc;
+    s_rank[tid] = rank;
+    s_start[tid] = start;
+    s_src[tid] = u;
+    if (tid == 0) {
+        const uint32_t base = atomic_add_u32(out.len, (uint32_t)total);
+        if ((uint64_t)base + (uint64_t)total > out.cap) st_relaxed(out.overflow, 1u);
+        g.o_start = (int32_t)base;
+    }
+    __syncthreads();
+    const uint32_t base = (uint32_t)g.o_start;
+    int lo = 0;
+    for (int p0 = 0; p0 < total; p0 += BFS_THREADS * EXPAND_ITEMS) {
+        int32_t col[EXPAND_ITEMS], src[EXPAND_ITEMS];
+#pragma unroll
+        for (int r = 0; r < EXPAND_ITEMS; ++r) {
+            const int p = p0 + r * BFS_THREADS + tid;
+            col[r] = -1;
+            if (p < total) {
+                // Last list with rank <= p; positions rise with r and p0.
+                int hi = BFS_THREADS;
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (s_rank[mid] <= p) lo = mid;
+                    else hi = mid;
+                }
+                col[r] = s_start[lo] + (p - s_rank[lo]);
+                src[r] = s_src[lo];
+            }
+        }
+        int32_t v[EXPAND_ITEMS];
+#pragma unroll
+        for (int r = 0; r < EXPAND_ITEMS; ++r) v[r] = col[r] >= 0 ? a.cols[col[r]] : 0;
+#pragma unroll
+        for (int r = 0; r < EXPAND_ITEMS; ++r) {
+            if (col[r] < 0) continue;
+            const uint64_t pos = (uint64_t)base + (uint32_t)(p0 + r * BFS_THREADS + tid);
+            if (pos < out.cap) {
+                out.keys[pos] = v[r];
+                out.vals[pos] = src[r];
+            }
+        }
+    }
+    __syncthreads();
+}
+
 // Two-phase expand (vertex frontier -> edge frontier of (neighbour, pred)),
-// two_phase/expand_atomic/cta.cuh:606-719.  No culling.
+// two_phase/expand_atomic/cta.cuh:606-719.  No culling: lists below the
+// grid threshold are expanded in place per tile, longer ones by k_heavy.
 __device__ void tp_expand_phase(const BfsArgs& a, int ph, LevelShared& s) {
     const uint32_t n_in = phase_input_len(a, ph);
     phase_begin(a, ph, n_in);
     const int32_t* qin = a.qk[ph & 1];
     const GlobalQueue out = out_queue(a, ph);
-    PairSink sink{s, out};
     for (uint32_t base = blockIdx.x * BFS_THREADS; base < n_in; base += gridDim.x * BFS_THREADS) {
         const uint32_t i = base + threadIdx.x;
         int32_t u = -1, start = 0, deg = 0;
@@ -298,10 +359,8 @@ __device__ void tp_expand_phase(const BfsArgs& a, int ph, LevelShared& s) {
             }
         }
         defer_heavy(a, ph, u, deg, a.grid_th);
-        tiered_gather(a.cols, u, start, deg, a.cta_th, a.wave_th, s.g, sink);
-        lq_flush(s.qp, out, LQ_CAP / 2);
+        expand_tile_in_place(a, out, u, start, deg, s.g);
     }
-    lq_flush(s.qp, out, 1);
 }
 
 // Two-phase filter (edge frontier -> edge frontier), bitmask and history
