@@ -149,6 +149,7 @@ def bench_lattice(bfs, b4, hip, info, width=200, sources=4, steps=8):
         r = measure_strategy(bfs, b4, hip, g, srcs, e_trav, v_vis, s_id, steps, 1, tuning=t)
         out[name] = {"gteps": round(r["gteps"], 3), "ms_per_search": round(r["ms_per_search_enact"], 3),
                      "levels_mean": round(float(np.mean(levels)), 1),
+                     "us_per_level": round(1e3 * r["ms_per_search_enact"] / max(1.0, float(np.mean(levels))), 2),
                      "roofline_frac": round(r["achieved_gbs"] / HBM_PEAK_GBS, 4)}
         log("lattice %s: %s" % (name, out[name]))
     g.close()
