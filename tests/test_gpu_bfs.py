@@ -132,6 +132,19 @@ def test_gather_tiers(bfsmod, orc, sm):
 
 
 @pytest.mark.parametrize("sm", STRATEGY_MODES)
+def test_grid_tier_many_chunks_per_workgroup(bfsmod, orc, rmat16, sm):
+    """Grid tier with a low threshold on 2 or 3 workgroups: each workgroup
+    owns thousands of 256-edge chunks, so the pair tier reserves several
+    batches per workgroup and the last round of in-flight chunks is partial."""
+    g, off, cols, srcs = rmat16
+    for grid in (2, 3):
+        tuning = {"grid_gather_threshold": 16, "grid_size": grid, "bottom_up_grid_size": grid}
+        for s in srcs[:2]:
+            depth, pred, st = run(bfsmod, g, int(s), *sm, tuning=tuning)
+            check(orc, off, cols, int(s), depth, pred, st)
+
+
+@pytest.mark.parametrize("sm", STRATEGY_MODES)
 def test_reuse_problem_across_sources(bfsmod, orc, rmat16, sm):
     g, off, cols, srcs = rmat16
     strategy, mode = sm
