@@ -12,8 +12,14 @@
 //                 [--strategy=0..5] [--mode=default|fused|iterative]
 //                 [--i=<sources>] [--src=<vertex>] [--seed=<n>]
 //                 [--queue-sizing=<f>] [--instrumented] [--v2]
+//                 [--microbench=gather_lookup|cull]
 // Prints one line per source and a summary; exit status 0 iff every source
-// is CORRECT.  As in the reference: --instrumented runs the instrumented
+// is CORRECT.  --microbench runs the per-stage microbenchmark search instead
+// of an enactor, as the reference's microbench_bfs.cu / microbench_cull.cu
+// drivers do (test/bfs/microbench/microbench_*.cu: the same driver over
+// EnactorGatherLookup / EnactorCull): per source the queue lengths of every
+// iteration, the stage times, the labels checked against the CPU BFS, and in
+// cull mode the culling efficiency (microbench_cull.cu:313-331).  As in the reference: --instrumented runs the instrumented
 // enactors (average workgroup duty), each source's line carries total queued
 // and redundant work ((queued - edges visited) / edges visited,
 // test_bfs.cu:369-373), the summary the mean, deviation and cv of search
@@ -60,6 +66,7 @@ struct Options {
     std::string file;
     bool undirected = false;  // file graphs: add a back edge per edge (test_bfs.cu:182-185)
     int splice = 0;
+    std::string microbench;   // "", "gather_lookup" or "cull"
 };
 
 bool file_kind(const std::string& k) { return k == "market" || k == "metis" || k == "dimacs"; }
@@ -78,6 +85,10 @@ bool parse(int argc, char** argv, Options& o) {
         else if (!strcmp(a, "--instrumented")) o.instrumented = true;
         else if (!strcmp(a, "--v2")) o.v2 = true;
         else if (!strncmp(a, "--splice=", 9)) o.splice = atoi(a + 9);
+        else if (!strncmp(a, "--microbench=", 13)) {
+            o.microbench = a + 13;
+            if (o.microbench != "gather_lookup" && o.microbench != "cull") return false;
+        }
         else if (a[0] == '-') return false;
         else if (pos == 0) { o.kind = a; ++pos; }
         else if (o.kind == "rmat" && pos == 1) { o.scale = atoi(a); ++pos; }
@@ -208,6 +219,91 @@ int run(const Options& o, b40c_graph* g, const std::vector<int32_t>& offs, const
     return failures ? 1 : 0;
 }
 
+// Per-stage microbenchmark search per source (the microbench drivers' loop):
+// the reference's queue columns per iteration, the stage times, labels
+// validated against the CPU BFS.  Elapsed = the real lookup, gather set-up
+// and gather kernels (the bench passes are reported beside it).
+int run_microbench(const Options& o, b40c_graph* g, const std::vector<int32_t>& offs,
+                   const std::vector<int32_t>& cols) {
+    const int32_t n = (int32_t)offs.size() - 1;
+    const bool cull = o.microbench == "cull";
+    const int mode = cull ? B40C_MB_CULL : B40C_MB_GATHER_LOOKUP;
+    std::vector<int32_t> sources;
+    if (o.src >= 0) {
+        sources.push_back(o.src);
+    } else {
+        sources.resize(o.iterations);
+        sources.resize(orc_pick_sources(n, offs.data(), o.seed, o.iterations, sources.data()));
+    }
+    std::vector<int32_t> depth(n), ref(n);
+    std::vector<b40c_mb_level> lv(4096);
+    int failures = 0;
+    RunningStat st_depth, st_cull, st_rate;
+    for (int32_t src : sources) {
+        int32_t nl = 0;
+        b40c_status s = b40c_microbench_search(g, src, mode, depth.data(), lv.data(), (int32_t)lv.size(), &nl);
+        if (s == B40C_OK && nl > (int32_t)lv.size()) {
+            lv.resize((size_t)nl);
+            s = b40c_microbench_search(g, src, mode, depth.data(), lv.data(), (int32_t)lv.size(), &nl);
+        }
+        if (s != B40C_OK) {
+            printf("src %d: status %d (%s)\n", src, s, b40c_last_error_string());
+            ++failures;
+            continue;
+        }
+        printf(cull ? "Iteration, Bitmask cull, Label cull, Local cull, Compaction queue, Expansion queue, "
+                      "bench lookup ms, lookup ms, offsets ms, bench gather ms, gather ms\n"
+                    : "Iteration, Bitmask cull, Compaction queue, Expansion queue, "
+                      "bench lookup ms, lookup ms, offsets ms, bench gather ms, gather ms\n");
+        double ms = 0.0, bench_ms = 0.0;
+        long long unculled = 0;
+        int32_t sd = 0;
+        for (int32_t i = 0; i < nl; ++i) {
+            const b40c_mb_level& r = lv[i];
+            if (cull)
+                printf("%d, %lld, %lld, %lld, %lld, %lld, %.4f, %.4f, %.4f, %.4f, %.4f\n", r.iteration,
+                       (long long)r.bitmask_pass, (long long)r.label_pass, (long long)r.local_pass,
+                       (long long)r.discovered, (long long)r.gathered, r.bench_lookup_ms, r.lookup_ms, r.offsets_ms,
+                       r.bench_gather_ms, r.gather_ms);
+            else
+                printf("%d, %lld, %lld, %lld, %.4f, %.4f, %.4f, %.4f, %.4f\n", r.iteration,
+                       (long long)r.bitmask_pass, (long long)r.discovered, (long long)r.gathered, r.bench_lookup_ms,
+                       r.lookup_ms, r.offsets_ms, r.bench_gather_ms, r.gather_ms);
+            ms += r.lookup_ms + r.offsets_ms + r.gather_ms;
+            bench_ms += r.bench_lookup_ms + r.bench_gather_ms;
+            if (r.iteration > 0) unculled += cull ? r.local_pass : r.label_pass;
+            if (r.discovered > 0) sd = r.iteration + 1;
+        }
+        const int32_t ref_sd = orc_bfs(n, offs.data(), cols.data(), src, ref.data(), nullptr);
+        const bool ok = depth == ref && sd == ref_sd;
+        int64_t nodes_visited = 0;
+        const int64_t edges_visited = orc_edges_visited(n, offs.data(), ref.data(), &nodes_visited);
+        const double rate = ms > 0 ? edges_visited / (ms * 1000.0) : 0.0;
+        // Share of the redundant lookups that the bench culls removed: every
+        // vertex but the source needs one surviving copy, the other
+        // edges_visited - (nodes_visited - 1) lookups are redundant
+        // (microbench_cull.cu:327-331 divides by edges - nodes).
+        const int64_t redundant = edges_visited - (nodes_visited - 1);
+        const double eff = redundant > 0 ? 100.0 * double(edges_visited - unculled) / double(redundant) : 0.0;
+        printf("src %d: %s depth=%s search_depth=%d (ref %d) nodes_visited=%lld edges_visited=%lld "
+               "total_unculled=%lld culling_efficiency=%.4f%% elapsed=%.3f ms bench=%.3f ms rate=%.2f MiEdges/s\n",
+               src, ok ? "CORRECT" : "INCORRECT", depth == ref ? "ok" : "MISMATCH", sd, ref_sd,
+               (long long)nodes_visited, (long long)edges_visited, unculled, eff, ms, bench_ms, rate);
+        st_depth.add(sd);
+        st_cull.add(eff);
+        st_rate.add(rate);
+        failures += ok ? 0 : 1;
+    }
+    if (st_rate.count > 0) {
+        printf("  Summary after %lld test iterations (bias-corrected):\n", st_rate.count);
+        st_depth.print("Search depth");
+        st_cull.print("Culling efficiency %");
+        st_rate.print("Rate MiEdges/s");
+    }
+    printf("summary: microbench=%s sources=%zu failures=%d\n", o.microbench.c_str(), sources.size(), failures);
+    return failures ? 1 : 0;
+}
+
 template <bool INSTRUMENT>
 int run_strategy(const Options& o, b40c_graph* g, const std::vector<int32_t>& offs, const std::vector<int32_t>& cols) {
     switch (o.strategy) {
@@ -230,7 +326,7 @@ int main(int argc, char** argv) {
         fprintf(stderr, "usage: %s rmat <scale> [edge_factor] | grid3d <width> | market <file> | metis <file> "
                         "| dimacs <file> [--undirected] [--splice=N] [--strategy=0..5] "
                         "[--mode=default|fused|iterative] [--i=N] [--src=V] [--seed=S] [--queue-sizing=F] "
-                        "[--instrumented] [--v2]\n",
+                        "[--instrumented] [--v2] [--microbench=gather_lookup|cull]\n",
                 argv[0]);
         return 2;
     }
@@ -253,7 +349,8 @@ int main(int argc, char** argv) {
     b40c_graph_copy_to_host(g, offs.data(), cols.data());
     printf("graph %s: nodes=%d edges=%lld\n", o.kind.c_str(), n, (long long)m);
     int rc = 2;
-    rc = o.instrumented ? run_strategy<true>(o, g, offs, cols) : run_strategy<false>(o, g, offs, cols);
+    if (!o.microbench.empty()) rc = run_microbench(o, g, offs, cols);
+    else rc = o.instrumented ? run_strategy<true>(o, g, offs, cols) : run_strategy<false>(o, g, offs, cols);
     b40c_graph_destroy(g);
     return rc;
 }

@@ -123,3 +123,28 @@ def test_cpp_driver_instrumented_histogram(strategy):
             break
         disc += int(parts[3])
     assert disc == nv
+
+
+MB_CASES = [("rmat", "12", "--microbench=gather_lookup"), ("rmat", "12", "--microbench=cull"),
+            ("grid3d", "16", "--microbench=cull")]
+
+
+@pytest.mark.parametrize("case", MB_CASES, ids=["-".join(c).replace("--microbench=", "") for c in MB_CASES])
+def test_cpp_driver_microbench(case):
+    """--microbench: the reference's microbench_bfs / microbench_cull drivers
+    (the test_bfs loop over the per-stage microbenchmark search): per-iteration
+    queue table, labels CORRECT against the CPU BFS, culling efficiency in
+    [0, 100] % (microbench_cull.cu:313-331)."""
+    if not os.path.exists(DRIVER):
+        pytest.fail("C++ driver not built (run __graft_entry__.build())")
+    print("driver microbench case:", case, flush=True)
+    r = subprocess.run([DRIVER, *case, "--i=3", "--seed=6"], capture_output=True, text=True, timeout=120)
+    print(r.stdout[-2500:])
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "failures=0" in r.stdout and r.stdout.count(": CORRECT") == 3
+    assert r.stdout.count("Compaction queue, Expansion queue") == 3
+    effs = [float(ln.split("culling_efficiency=")[1].split("%")[0]) for ln in r.stdout.splitlines()
+            if ln.startswith("src ")]
+    assert len(effs) == 3 and all(0.0 <= e <= 100.0 for e in effs)
+    if "cull" in case[-1] and "gather" not in case[-1]:
+        assert min(effs) > 0.0
