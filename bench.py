@@ -495,7 +495,8 @@ def probe(args):
 def measured_traffic(workload, strategy):
     """HBM bytes per search from the committed counter summary of the same
     workload and strategy (two rocprofv3 --pmc passes, FETCH_SIZE and
-    WRITE_SIZE, each scaled by its reading for a copy of known size);
+    WRITE_SIZE, each scaled by its reading for a copy of known size), and the
+    L2 hit rate of its search kernels (a TCC_HIT_sum / TCC_MISS_sum pass);
     traffic is null when there is no summary for this configuration."""
     path = os.path.join(ROOT, "profiles", "rmat22_direction_optimizing_traffic.json")
     try:
@@ -505,8 +506,11 @@ def measured_traffic(workload, strategy):
         return {"traffic": None}
     if t.get("workload") != workload or t.get("strategy") != strategy:
         return {"traffic": None}
-    return {"traffic": t["traffic_bytes_per_search"], "traffic_unit": "bytes per search",
-            "traffic_source": "profiles/rmat22_direction_optimizing_traffic.json (commit %s)" % t.get("commit")}
+    out = {"traffic": t["traffic_bytes_per_search"], "traffic_unit": "bytes per search",
+           "traffic_source": "profiles/rmat22_direction_optimizing_traffic.json (commit %s)" % t.get("commit")}
+    if "l2_hit_rate" in t:
+        out["l2_hit_rate"] = t["l2_hit_rate"]
+    return out
 
 
 def run_child(which, timeout_s=420):
