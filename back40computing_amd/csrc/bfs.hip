@@ -406,7 +406,10 @@ __device__ void tp_filter_phase(const BfsArgs& a, int ph, LevelShared& s) {
 // A wave owns the two bitmap words of its 64 vertices, so visited/next words
 // are written without atomics.  Discovered vertices go to the vertex queue
 // (one global atomic per wave tile) so that a top-down step can follow.
-constexpr int BU_FIRST = 4;
+// Measured on RMAT-22 (ms per search): BU_FIRST 1-3 within 1 % of each
+// other, 4 about 2 % slower, 6 and 8 4-10 % slower; BU_GROUP 4 and 16 no
+// better than 8.
+constexpr int BU_FIRST = 2;
 constexpr int BU_GROUP = 8;
 constexpr int BU_WAVE_GROUPS = kWave / BU_GROUP;
 
