@@ -19,8 +19,11 @@
 // drivers do (test/bfs/microbench/microbench_*.cu: the same driver over
 // EnactorGatherLookup / EnactorCull): per source the queue lengths of every
 // iteration, the stage times, the labels checked against the CPU BFS, and in
-// cull mode the culling efficiency (microbench_cull.cu:313-331).  As in the reference: --instrumented runs the instrumented
-// enactors (average workgroup duty), each source's line carries total queued
+// cull mode the culling efficiency (microbench_cull.cu:313-331), through
+// the facade's microbench::EnactorGatherLookup / EnactorCull
+// (include/b40c/microbench.hpp).  As in the reference: --instrumented runs
+// the instrumented enactors (average workgroup duty), each source's line
+// carries total queued
 // and redundant work ((queued - edges visited) / edges visited,
 // test_bfs.cu:369-373), the summary the mean, deviation and cv of search
 // depth, redundant work, duty and rate (test_bfs.cu:451-478), and --v2 prints
@@ -36,6 +39,7 @@
 
 #include "b40c/bfs.hpp"
 #include "b40c/graph_builder.hpp"
+#include "b40c/microbench.hpp"
 
 extern "C" {
 int32_t orc_bfs(int32_t n, const int32_t* offsets, const int32_t* cols, int32_t src, int32_t* depth, int32_t* pred);
@@ -223,11 +227,12 @@ int run(const Options& o, b40c_graph* g, const std::vector<int32_t>& offs, const
 // the reference's queue columns per iteration, the stage times, labels
 // validated against the CPU BFS.  Elapsed = the real lookup, gather set-up
 // and gather kernels (the bench passes are reported beside it).
+template <typename MbEnactor>
 int run_microbench(const Options& o, b40c_graph* g, const std::vector<int32_t>& offs,
                    const std::vector<int32_t>& cols) {
     const int32_t n = (int32_t)offs.size() - 1;
     const bool cull = o.microbench == "cull";
-    const int mode = cull ? B40C_MB_CULL : B40C_MB_GATHER_LOOKUP;
+    MbEnactor enactor;
     std::vector<int32_t> sources;
     if (o.src >= 0) {
         sources.push_back(o.src);
@@ -236,16 +241,12 @@ int run_microbench(const Options& o, b40c_graph* g, const std::vector<int32_t>& 
         sources.resize(orc_pick_sources(n, offs.data(), o.seed, o.iterations, sources.data()));
     }
     std::vector<int32_t> depth(n), ref(n);
-    std::vector<b40c_mb_level> lv(4096);
     int failures = 0;
     RunningStat st_depth, st_cull, st_rate;
     for (int32_t src : sources) {
-        int32_t nl = 0;
-        b40c_status s = b40c_microbench_search(g, src, mode, depth.data(), lv.data(), (int32_t)lv.size(), &nl);
-        if (s == B40C_OK && nl > (int32_t)lv.size()) {
-            lv.resize((size_t)nl);
-            s = b40c_microbench_search(g, src, mode, depth.data(), lv.data(), (int32_t)lv.size(), &nl);
-        }
+        const b40c_status s = enactor.EnactSearch(g, src, depth.data());
+        const std::vector<b40c_mb_level>& lv = enactor.Levels();
+        const int32_t nl = (int32_t)lv.size();
         if (s != B40C_OK) {
             printf("src %d: status %d (%s)\n", src, s, b40c_last_error_string());
             ++failures;
@@ -256,8 +257,10 @@ int run_microbench(const Options& o, b40c_graph* g, const std::vector<int32_t>& 
                     : "Iteration, Bitmask cull, Compaction queue, Expansion queue, "
                       "bench lookup ms, lookup ms, offsets ms, bench gather ms, gather ms\n");
         double ms = 0.0, bench_ms = 0.0;
-        long long unculled = 0;
+        long long unculled = 0, total = 0;
         int32_t sd = 0;
+        double expand_duty = 0.0, compact_duty = 0.0;
+        enactor.GetStatistics(total, sd, expand_duty, compact_duty);
         for (int32_t i = 0; i < nl; ++i) {
             const b40c_mb_level& r = lv[i];
             if (cull)
@@ -272,7 +275,6 @@ int run_microbench(const Options& o, b40c_graph* g, const std::vector<int32_t>& 
             ms += r.lookup_ms + r.offsets_ms + r.gather_ms;
             bench_ms += r.bench_lookup_ms + r.bench_gather_ms;
             if (r.iteration > 0) unculled += cull ? r.local_pass : r.label_pass;
-            if (r.discovered > 0) sd = r.iteration + 1;
         }
         const int32_t ref_sd = orc_bfs(n, offs.data(), cols.data(), src, ref.data(), nullptr);
         const bool ok = depth == ref && sd == ref_sd;
@@ -286,9 +288,10 @@ int run_microbench(const Options& o, b40c_graph* g, const std::vector<int32_t>& 
         const int64_t redundant = edges_visited - (nodes_visited - 1);
         const double eff = redundant > 0 ? 100.0 * double(edges_visited - unculled) / double(redundant) : 0.0;
         printf("src %d: %s depth=%s search_depth=%d (ref %d) nodes_visited=%lld edges_visited=%lld "
-               "total_unculled=%lld culling_efficiency=%.4f%% elapsed=%.3f ms bench=%.3f ms rate=%.2f MiEdges/s\n",
+               "%s=%lld culling_efficiency=%.4f%% elapsed=%.3f ms bench=%.3f ms rate=%.2f MiEdges/s\n",
                src, ok ? "CORRECT" : "INCORRECT", depth == ref ? "ok" : "MISMATCH", sd, ref_sd,
-               (long long)nodes_visited, (long long)edges_visited, unculled, eff, ms, bench_ms, rate);
+               (long long)nodes_visited, (long long)edges_visited, cull ? "total_unculled" : "total_queued", total,
+               eff, ms, bench_ms, rate);
         st_depth.add(sd);
         st_cull.add(eff);
         st_rate.add(rate);
@@ -349,7 +352,9 @@ int main(int argc, char** argv) {
     b40c_graph_copy_to_host(g, offs.data(), cols.data());
     printf("graph %s: nodes=%d edges=%lld\n", o.kind.c_str(), n, (long long)m);
     int rc = 2;
-    if (!o.microbench.empty()) rc = run_microbench(o, g, offs, cols);
+    if (o.microbench == "cull") rc = run_microbench<b40c::graph::bfs::microbench::EnactorCull>(o, g, offs, cols);
+    else if (!o.microbench.empty())
+        rc = run_microbench<b40c::graph::bfs::microbench::EnactorGatherLookup>(o, g, offs, cols);
     else rc = o.instrumented ? run_strategy<true>(o, g, offs, cols) : run_strategy<false>(o, g, offs, cols);
     b40c_graph_destroy(g);
     return rc;
