@@ -133,6 +133,16 @@ def test_many_levels_and_truncated_records(orc, mb):
     assert [recs[i].iteration for i in range(10)] == list(range(10))
 
 
+def test_edgeless_graph(orc, mb):
+    """No edges at all: every search is the source alone, one record."""
+    from back40computing_amd import bfs
+    g = bfs.Graph.from_coo(5, np.zeros(0, np.int32), np.zeros(0, np.int32))
+    for mode in ("gather_lookup", "cull"):
+        depth, levels = mb.search(g, 3, mode)
+        assert depth.tolist() == [-1, -1, -1, 0, -1]
+        assert len(levels) == 1 and levels[0]["discovered"] == 1 and levels[0]["gathered"] == 0
+
+
 def test_bad_arguments(mb):
     from back40computing_amd import B40CError, bfs
     g = bfs.Graph.rmat(8, 4, 0.57, 0.19, 0.19, 1)
