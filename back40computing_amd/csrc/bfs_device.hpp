@@ -21,6 +21,8 @@
 //    two_phase/contract_atomic/cta.cuh:276-297, sized for wave64 + LDS).
 #pragma once
 
+#include <cstddef>
+
 #include "common.hpp"
 
 namespace b40c {
@@ -46,6 +48,7 @@ struct BfsArgs {
     int32_t* qv[2];          // frontier queue values (predecessors), edge frontiers only
     uint32_t qcap[2];        // capacities of qk[i] (elements)
     DevCounters* ctr;
+    uint32_t* bar_arrive;    // persistent kernels: grid-barrier arrival word per workgroup
     DevCounters* hctr;       // device address of the pinned host mirror of *ctr
     uint32_t* max_depth;     // deepest depth label written
     unsigned long long* wg_stats;  // instrumented: [grid][2] (busy clocks, lifetime clocks)
@@ -67,15 +70,23 @@ constexpr int HEAVY_CHUNK = 256;     // grid-tier edges per work item (one per t
 constexpr int HEAVY_ROUND = 4;       // grid-tier work items a workgroup has in flight at once
 
 // ---------------------------------------------------------------------------
-// LDS-buffered queue.
+// LDS-buffered queue.  VCAP is the capacity of the value array: LQ_CAP for a
+// queue that can carry (key, value) pairs, 1 for a keys-only queue.  A
+// pair-capable queue also serves keys-only appends (VALS = false), so one
+// kernel that alternates key and pair steps uses ONE queue object; no other
+// field ever shares storage with `count` or `gbase`.
 // ---------------------------------------------------------------------------
-template <bool VALS>
-struct LdsQueue {
+template <int VCAP>
+struct LdsQueueT {
     int32_t keys[LQ_CAP];
-    int32_t vals[VALS ? LQ_CAP : 1];
+    int32_t vals[VCAP];
     uint32_t count;
     uint32_t gbase;
 };
+using LdsKeyQueue = LdsQueueT<1>;
+using LdsPairQueue = LdsQueueT<LQ_CAP>;
+static_assert(offsetof(LdsKeyQueue, count) >= sizeof(int32_t) * (LQ_CAP + 1), "key queue count overlaps its arrays");
+static_assert(offsetof(LdsPairQueue, count) >= sizeof(int32_t) * 2 * LQ_CAP, "pair queue count overlaps its arrays");
 
 struct GlobalQueue {
     int32_t* keys;
@@ -86,8 +97,10 @@ struct GlobalQueue {
 };
 
 // Wave-uniform call: every lane of the wave must be active.
-template <bool VALS>
-__device__ __forceinline__ void lq_append(LdsQueue<VALS>& q, const GlobalQueue& g, bool want, int32_t k, int32_t v) {
+template <bool VALS, int VCAP>
+__device__ __forceinline__ void lq_append(LdsQueueT<VCAP>& q, const GlobalQueue& g, bool want, int32_t k,
+                                          int32_t v) {
+    static_assert(!VALS || VCAP == LQ_CAP, "pair appends need a pair queue");
     const uint64_t m = __ballot(want);
     if (m == 0) return;
     const int lane = lane_id();
@@ -121,8 +134,9 @@ __device__ __forceinline__ void lq_append(LdsQueue<VALS>& q, const GlobalQueue& 
 
 // Workgroup-uniform call.  Flushes when the buffer holds at least `min_fill`
 // entries (min_fill = 1: flush whatever is there).
-template <bool VALS>
-__device__ __forceinline__ void lq_flush(LdsQueue<VALS>& q, const GlobalQueue& g, uint32_t min_fill) {
+template <bool VALS, int VCAP>
+__device__ __forceinline__ void lq_flush(LdsQueueT<VCAP>& q, const GlobalQueue& g, uint32_t min_fill) {
+    static_assert(!VALS || VCAP == LQ_CAP, "pair flushes need a pair queue");
     __syncthreads();
     uint32_t cnt = q.count;
     __syncthreads();

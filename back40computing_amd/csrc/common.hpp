@@ -50,8 +50,8 @@ struct alignas(256) DevCounters {
     unsigned long long queued;   // total elements taken from input frontiers
     unsigned long long next_edges;  // degree sum of the vertices discovered this level
     unsigned long long next_edges_prev;
-    uint32_t bar_count;          // grid barrier arrival counter
-    uint32_t bar_gen;            // grid barrier generation
+    uint32_t bar_pad;
+    uint32_t bar_release;        // grid barrier: epoch of the last completed barrier (never reset by a search)
     uint32_t bar_abort;          // set when a barrier wait timed out
     uint32_t corrupt;            // set when a frontier held an id outside [0, n)
     unsigned long long level_edges[4];  // rotating per-level degree sums (direction switch)
@@ -137,42 +137,74 @@ __device__ __forceinline__ int wg_exclusive_scan(int x, int& total, int* s_wave)
 
 // ---------------------------------------------------------------------------
 // Software grid barrier for cooperatively launched (co-resident) grids
-// (replaces GlobalBarrier, b40c/util/global_barrier.cuh:47-127).  A counter
-// plus a generation word: the last workgroup to arrive resets the counter and
-// bumps the generation with release semantics; the others spin with acquire
-// loads.  Agent scope makes the level's writes visible across all XCDs' L2s.
-// ---------------------------------------------------------------------------
+// (replaces GlobalBarrier, b40c/util/global_barrier.cuh:47-127).
+//
+// Each workgroup except 0 owns one arrival word and stores the barrier's
+// epoch there with release semantics, then polls the release word.
+// Workgroup 0 polls every arrival word at once (each of its threads loads a
+// strided share of them per poll, all loads in flight together), then stores
+// the epoch into the release word.  No two workgroups ever update the same
+// word, so there is no same-address atomic traffic; a barrier costs about
+// three agent-scope round trips (arrival store, master poll, release poll).
+//
+// Epochs grow monotonically over the lifetime of the barrier memory: a
+// kernel reads *release once at its start (every workgroup reads the same
+// value, since no barrier can complete before all have arrived) and passes
+// base + 1, base + 2, ... to successive barriers.  Arrival words therefore
+// only ever hold epochs up to the current one, and equality means "arrived".
+// Agent scope makes the level's writes visible across all XCDs' L2s.
+//
 // The wait is bounded: after kBarrierSpinLimit polls a workgroup raises
 // `abort_flag` and leaves, and waiters leave as soon as they see the flag, so
 // a broken launch (for example a grid that is not co-resident) ends instead
-// of hanging; callers stop their level loop when the flag is set.
-constexpr uint32_t kBarrierSpinLimit = 1u << 24;
+// of hanging.  grid_sync returns false in every thread of every workgroup
+// that saw the abort; callers stop their level loop then.  After an abort
+// the epochs of the workgroups disagree, so the host clears the barrier
+// memory before its next cooperative launch.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kBarrierSpinLimit = 1u << 22;
 
-__device__ __forceinline__ void grid_barrier(uint32_t* count, uint32_t* gen, uint32_t nblocks, uint32_t* abort_flag) {
+struct GridBarrier {
+    uint32_t* arrive;      // [grid] arrival epoch of each workgroup (word 0 unused)
+    uint32_t* release;     // epoch of the last completed barrier
+    uint32_t* abort_flag;  // nonzero once a wait ran out of polls
+};
+
+__device__ __forceinline__ bool grid_sync(const GridBarrier& b, uint32_t epoch) {
     __syncthreads();
-    if (threadIdx.x == 0) {
-        const uint32_t g = ld_relaxed(gen);
-        const uint32_t arrived = __hip_atomic_fetch_add(count, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        if (arrived == nblocks - 1) {
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            st_relaxed(count, 0u);
-            __hip_atomic_store(gen, g + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            // Poll with relaxed loads (no cache invalidation per poll), then
-            // one acquire fence once the generation has moved.
-            uint32_t spins = 0;
-            while (ld_relaxed(gen) == g) {
-                if (ld_relaxed(abort_flag)) break;
-                if (++spins > kBarrierSpinLimit) {
-                    st_relaxed(abort_flag, 1u);
-                    break;
-                }
-                __builtin_amdgcn_s_sleep(1);
+    bool aborted = false;
+    const uint32_t nb = gridDim.x;
+    if (blockIdx.x == 0) {
+        uint32_t spins = 0;
+        while (true) {
+            bool all = true;
+            for (uint32_t w = 1 + threadIdx.x; w < nb; w += blockDim.x) all &= ld_relaxed(&b.arrive[w]) == epoch;
+            if (all) break;
+            if (ld_relaxed(b.abort_flag) || ++spins > kBarrierSpinLimit) {
+                st_relaxed(b.abort_flag, 1u);
+                aborted = true;
+                break;
             }
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            __builtin_amdgcn_s_sleep(1);
         }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        aborted = __syncthreads_or(aborted) != 0;
+        if (threadIdx.x == 0 && !aborted)
+            __hip_atomic_store(b.release, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else if (threadIdx.x == 0) {
+        __hip_atomic_store(&b.arrive[blockIdx.x], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t spins = 0;
+        while (ld_relaxed(b.release) != epoch) {
+            if (ld_relaxed(b.abort_flag) || ++spins > kBarrierSpinLimit) {
+                st_relaxed(b.abort_flag, 1u);
+                aborted = true;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
-    __syncthreads();
+    return __syncthreads_or(aborted) == 0;
 }
 
 // ---------------------------------------------------------------------------

@@ -110,7 +110,7 @@ __global__ void __launch_bounds__(MB_THREADS) k_mb_lookup(const int32_t* __restr
                                                           uint32_t* mask, int32_t* labels, int32_t iteration,
                                                           int32_t* pair_off, int32_t* pair_len, uint32_t pair_cap,
                                                           MbCounters* ctr) {
-    __shared__ LdsQueue<true> q;
+    __shared__ LdsPairQueue q;
     __shared__ uint32_t red32[3][BFS_WAVES];
     __shared__ unsigned long long red64[BFS_WAVES];
     if (threadIdx.x == 0) q.count = 0;
@@ -150,10 +150,10 @@ __global__ void __launch_bounds__(MB_THREADS) k_mb_lookup(const int32_t* __restr
             edges += (unsigned long long)len[r];
         }
 #pragma unroll
-        for (int r = 0; r < LK_ITEMS; ++r) lq_append(q, out, keep[r] && len[r] > 0, start[r], len[r]);
-        lq_flush(q, out, LQ_CAP / 2);
+        for (int r = 0; r < LK_ITEMS; ++r) lq_append<true>(q, out, keep[r] && len[r] > 0, start[r], len[r]);
+        lq_flush<true>(q, out, LQ_CAP / 2);
     }
-    lq_flush(q, out, 1);
+    lq_flush<true>(q, out, 1);
     const uint32_t c[3] = {found, 0u, 0u};
     wg_add3(c, &ctr->discovered, red32, 1);
     wg_add_u64(edges, &ctr->next_edges, red64);

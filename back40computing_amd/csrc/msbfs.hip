@@ -127,7 +127,7 @@ constexpr int MS_TILE_TABLE = 512;
 constexpr int MS_TILE = BFS_THREADS;  // vertices per pull tile
 
 struct MsShared {
-    LdsQueue<false> q;
+    LdsKeyQueue q;
     GatherSmem g;
     unsigned long long red[BFS_WAVES];
 };
@@ -177,7 +177,7 @@ __device__ __forceinline__ void ms_push_edge(const MsArgs& a, int ph, MsShared& 
         ms_record(a, ph, v, src, nb);
         enq = atomic_or_u64(&a.front[(ph + 1) % 3][v], nb) == 0ull;
     }
-    lq_append(s.q, out, enq, v, 0);
+    lq_append<false>(s.q, out, enq, v, 0);
 }
 
 // Grid tier: a list of at least grid_th edges is cut into HEAVY_CHUNK-edge
@@ -268,9 +268,9 @@ __global__ __launch_bounds__(BFS_THREADS) void k_ms_push(MsArgs a, int ph) {
         ms_defer_heavy(a, ph, u, deg);
         auto sink = [&](bool valid, int32_t v, int32_t src) { ms_push_edge(a, ph, s, out, valid, v, src); };
         tiered_gather(a.cols, u, start, deg, a.cta_th, a.wave_th, s.g, sink);
-        lq_flush(s.q, out, LQ_CAP / 2);
+        lq_flush<false>(s.q, out, LQ_CAP / 2);
     }
-    lq_flush(s.q, out, 1);
+    lq_flush<false>(s.q, out, 1);
 }
 
 // Bottom-up pull of level ph for the bits in bottom-up mode: each vertex of
@@ -289,7 +289,7 @@ constexpr int MS_GROUP = 8;
 constexpr int MS_GROUPS = BFS_THREADS / MS_GROUP;
 
 struct MsPullShared {
-    LdsQueue<false> q;
+    LdsKeyQueue q;
     int2 tab[MS_TILE_TABLE];  // (graph, first tile) of the graphs with bottom-up slots
     int32_t pv[BFS_THREADS];  // unresolved vertex (tile-local index)
     int32_t pe[BFS_THREADS];  // next edge to scan
@@ -434,10 +434,10 @@ __global__ __launch_bounds__(BFS_THREADS) void k_ms_pull(MsArgs a, int ph) {
             atomic_or_u64(&a.visited[v], got);
             enq = atomic_or_u64(&next[v], got) == 0ull;
         }
-        lq_append(s.q, out, enq, v, 0);
-        lq_flush(s.q, out, LQ_CAP / 2);
+        lq_append<false>(s.q, out, enq, v, 0);
+        lq_flush<false>(s.q, out, LQ_CAP / 2);
     }
-    lq_flush(s.q, out, 1);
+    lq_flush<false>(s.q, out, 1);
 }
 
 // Grid tier of level ph's push: workgroups take chunks grid-stride, one edge
@@ -473,9 +473,9 @@ __global__ __launch_bounds__(BFS_THREADS) void k_ms_heavy(MsArgs a, int ph) {
         }
         const int32_t v = valid ? a.cols[e] : -1;
         ms_push_edge(a, ph, s, out, valid, v, u);
-        lq_flush(s.q, out, LQ_CAP / 2);
+        lq_flush<false>(s.q, out, LQ_CAP / 2);
     }
-    lq_flush(s.q, out, 1);
+    lq_flush<false>(s.q, out, 1);
 }
 
 // Per (graph, slot) count and degree sum of the vertices discovered by level

@@ -12,10 +12,6 @@ pytestmark = pytest.mark.gpu
 
 import back40computing_amd as b4  # noqa: E402
 
-# The persistent (cooperative, grid-barrier) kernels are switched off in the
-# library until the cause of a GPU memory fault seen in a run that used them
-# is shown; their cases are skipped and the refusal itself is tested below.
-FUSED_SKIP = pytest.mark.skip(reason="persistent kernels disabled pending the cause of a GPU fault")
 ALL_MODES = [
     (b4.EXPAND_CONTRACT, b4.MODE_ITERATIVE),
     (b4.EXPAND_CONTRACT, b4.MODE_FUSED),
@@ -29,20 +25,8 @@ ALL_MODES = [
 ]
 
 
-def is_fused(sm):
-    return sm[1] == b4.MODE_FUSED or sm[0] == b4.PERSISTENT
-
-
-def params(held_back=()):
-    out = []
-    for sm in ALL_MODES:
-        marks = []
-        if is_fused(sm):
-            marks.append(FUSED_SKIP)
-        elif sm in held_back:
-            marks.append(pytest.mark.skip(reason="held back pending the cause of a GPU fault"))
-        out.append(pytest.param(sm, marks=marks, id="%d-%d" % sm))
-    return out
+def params():
+    return [pytest.param(sm, id="%d-%d" % sm) for sm in ALL_MODES]
 
 
 STRATEGY_MODES = params()
@@ -94,12 +78,7 @@ def test_rmat16_all_strategies(bfsmod, orc, rmat16, sm):
         check(orc, off, cols, s, depth, pred, st)
 
 
-# Two-phase on the lattice is one of the two cases that may have faulted the
-# GPU (the run's log was lost with the box); it stays off until a cause is shown.
-LATTICE_MODES = params(held_back=[(b4.TWO_PHASE, b4.MODE_ITERATIVE)])
-
-
-@pytest.mark.parametrize("sm", LATTICE_MODES)
+@pytest.mark.parametrize("sm", STRATEGY_MODES)
 def test_lattice_closed_form(bfsmod, orc, sm):
     w = 24
     g, perm = bfsmod.Graph.lattice3d(w, 5, want_perm=True)
@@ -179,7 +158,10 @@ def test_enact_requires_reset(bfsmod, rmat16):
 @pytest.mark.parametrize("strategy,mode", [(b4.CONTRACT_EXPAND, b4.MODE_ITERATIVE),
                                            (b4.TWO_PHASE, b4.MODE_ITERATIVE),
                                            (b4.HYBRID, b4.MODE_DEFAULT),
-                                           (b4.EXPAND_CONTRACT, b4.MODE_ITERATIVE)])
+                                           (b4.EXPAND_CONTRACT, b4.MODE_ITERATIVE),
+                                           (b4.CONTRACT_EXPAND, b4.MODE_FUSED),
+                                           (b4.TWO_PHASE, b4.MODE_FUSED),
+                                           (b4.EXPAND_CONTRACT, b4.MODE_FUSED)])
 def test_forced_queue_overflow(bfsmod, rmat16, strategy, mode):
     """A tiny queue-sizing factor must surface as QUEUE_OVERFLOW, never as
     silent corruption (enactor_contract_expand.cuh:319-324)."""
@@ -206,21 +188,27 @@ def test_statistics_and_instrumentation(bfsmod, orc, rmat16):
         assert q >= int((ref >= 0).sum())
 
 
-def test_fused_modes_refused(bfsmod, rmat16):
-    """While the persistent kernels are off, every fused request is refused
-    on the host with UNSUPPORTED (no kernel is launched)."""
+def test_fused_entry_points(bfsmod, orc, rmat16):
+    """EnactFusedSearch of each enactor class and the persistent enactor's
+    EnactSearch: one cooperative launch each, labels equal to the oracle's,
+    and the problem reused for a second source (the grid barrier's epochs
+    continue from the first search)."""
     g, off, cols, srcs = rmat16
-    for cls in (bfsmod.EnactorExpandContract, bfsmod.EnactorContractExpand, bfsmod.EnactorTwoPhase):
+    for cls in (bfsmod.EnactorExpandContract, bfsmod.EnactorContractExpand, bfsmod.EnactorTwoPhase,
+                bfsmod.EnactorPersistent):
         prob = bfsmod.CsrProblem()
         assert prob.FromGraph(g) == 0
         en = cls()
-        assert prob.Reset(en.GetFrontierType()) == 0
-        assert en.EnactFusedSearch(prob, int(srcs[0])) == b4._lib.ERR_UNSUPPORTED
-    prob = bfsmod.CsrProblem()
-    assert prob.FromGraph(g) == 0
-    en = bfsmod.EnactorPersistent()
-    assert prob.Reset(en.GetFrontierType()) == 0
-    assert en.EnactSearch(prob, int(srcs[0])) == b4._lib.ERR_UNSUPPORTED
+        for s in srcs[:2]:
+            print("fused entry %s src=%d" % (cls.__name__, int(s)), flush=True)
+            assert prob.Reset(en.GetFrontierType()) == 0
+            if cls is bfsmod.EnactorPersistent:
+                assert en.EnactSearch(prob, int(s)) == 0
+            else:
+                assert en.EnactFusedSearch(prob, int(s)) == 0
+            depth, pred = prob.results()
+            q, sd, duty = en.GetStatistics()
+            check(orc, off, cols, s, depth, pred, {"search_depth": sd})
 
 
 def test_gbarrier_zz_visibility(bfsmod):
@@ -228,16 +216,17 @@ def test_gbarrier_zz_visibility(bfsmod):
     workgroups after it (indices are fixed and in bounds; values are only
     compared)."""
     info = bfsmod.device_info()
-    for grid in (info["num_cus"], 2 * info["num_cus"]):
+    for grid in (info["num_cus"], 2 * info["num_cus"], 6 * info["num_cus"]):
         print("gbarrier_check grid=%d" % grid, flush=True)
         assert bfsmod.gbarrier_check(grid, 200) == 0
 
 
 def test_gbarrier_zz_benchmark(bfsmod):
     info = bfsmod.device_info()
-    ms = bfsmod.gbarrier_benchmark(info["num_cus"], 1000)
-    print("gbarrier %.4f ms/sync at grid %d" % (ms, info["num_cus"]), flush=True)
-    assert 0 < ms < 1.0
+    for grid in (info["num_cus"], 2 * info["num_cus"], 6 * info["num_cus"]):
+        ms = bfsmod.gbarrier_benchmark(grid, 2000)
+        print("gbarrier %.2f us/sync at grid %d" % (ms * 1e3, grid), flush=True)
+        assert 0 < ms < 1.0
 
 
 def test_from_host_problem_signature(bfsmod, orc):
