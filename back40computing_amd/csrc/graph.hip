@@ -477,6 +477,17 @@ b40c_status b40c_graph_from_host_csr(int32_t nodes, int64_t edges, const int32_t
     if (row_offsets[0] != 0 || row_offsets[nodes] != edges)
         return fail(B40C_ERR_INVALID_ARG, "b40c_graph_from_host_csr: offsets[0]=%d offsets[n]=%d edges=%lld",
                     row_offsets[0], row_offsets[nodes], (long long)edges);
+    // The kernels use neighbour ids and row ranges as addresses, so a CSR
+    // whose offsets decrease or whose column ids leave [0, n) is refused
+    // here instead of turning into out-of-bounds device accesses.
+    for (int32_t v = 0; v < nodes; ++v)
+        if (row_offsets[v + 1] < row_offsets[v])
+            return fail(B40C_ERR_INVALID_ARG, "b40c_graph_from_host_csr: offsets decrease at vertex %d (%d > %d)", v,
+                        row_offsets[v], row_offsets[v + 1]);
+    for (int64_t e = 0; e < edges; ++e)
+        if ((uint32_t)column_indices[e] >= (uint32_t)nodes)
+            return fail(B40C_ERR_INVALID_ARG, "b40c_graph_from_host_csr: column id %d at entry %lld is out of range [0,%d)",
+                        column_indices[e], (long long)e, nodes);
     b40c_graph* g = new b40c_graph();
     g->nodes = nodes;
     g->edges = edges;

@@ -727,6 +727,9 @@ __global__ void k_pack_offs(const int32_t* offs, int32_t n, int32_t eshift, int3
 // ===========================================================================
 
 struct b40c_multigraph {
+    // Reference counted: each batched engine built on the pack holds a
+    // reference, so destroying the pack before its engines is safe.
+    std::atomic<int> refs{1};
     b40c_graph* packed = nullptr;
     int32_t ngraphs = 0;
     std::vector<int32_t> vbase;  // ngraphs + 1
@@ -800,6 +803,7 @@ struct b40c_msbfs {
     uint32_t qcap = 0, heavy_cap = 0, chunk_cap = 0;
     b40c::ArenaAlloc arena;
     std::vector<b40c::MsQuery> queries;
+    std::vector<int32_t> free_ids;  // ids whose results were taken, reused by later submissions
     std::deque<int32_t> waiting;
     std::vector<unsigned long long> h_active;  // [G]
     std::vector<int32_t> running;              // query ids in flight
@@ -1053,6 +1057,7 @@ b40c_status b40c_multigraph_info(const b40c_multigraph* mg, int32_t* ngraphs, in
 
 void b40c_multigraph_destroy(b40c_multigraph* mg) {
     if (!mg) return;
+    if (mg->refs.fetch_sub(1) != 1) return;
     graph_release(mg->packed);
     delete mg;
 }
@@ -1067,6 +1072,7 @@ b40c_status b40c_msbfs_create(b40c_multigraph* mg, int64_t label_budget_bytes, i
         return fail(B40C_ERR_UNSUPPORTED, "b40c_msbfs_create: the batched engine needs symmetric graphs "
                                           "(bottom-up steps pull from neighbours)");
     b40c_msbfs* m = new b40c_msbfs();
+    mg->refs.fetch_add(1);
     m->mg = mg;
     m->n = mg->packed->nodes;
     m->G = mg->ngraphs;
@@ -1153,13 +1159,22 @@ b40c_status b40c_msbfs_submit(b40c_msbfs* m, int32_t graph, int32_t src, int32_t
     if ((int64_t)vg > m->arena_elems)
         return fail(B40C_ERR_ADMISSION, "b40c_msbfs_submit: labels of graph %d (%d vertices) exceed the budget", graph,
                     vg);
-    if ((int32_t)m->queries.size() >= m->qmax)
-        return fail(B40C_ERR_ADMISSION, "b40c_msbfs_submit: more than %d queries", m->qmax);
+    // Ids are recycled once a query's result has been taken, so max_queries
+    // bounds the queries submitted and not yet collected, not the total.
+    int32_t q = -1;
+    if (!m->free_ids.empty()) {
+        q = m->free_ids.back();
+        m->free_ids.pop_back();
+    } else if ((int32_t)m->queries.size() < m->qmax) {
+        m->queries.emplace_back();
+        q = (int32_t)m->queries.size() - 1;
+    } else {
+        return fail(B40C_ERR_ADMISSION, "b40c_msbfs_submit: %d queries submitted and not yet collected", m->qmax);
+    }
     MsQuery Q;
     Q.graph = graph;
     Q.src = src;
-    m->queries.push_back(Q);
-    const int32_t q = (int32_t)m->queries.size() - 1;
+    m->queries[q] = Q;
     m->waiting.push_back(q);
     *query_id = q;
     return B40C_OK;
@@ -1229,6 +1244,7 @@ b40c_status b40c_msbfs_result(b40c_msbfs* m, int32_t q, int32_t* depth, int32_t*
     if (search_depth) *search_depth = Q.levels;
     m->arena.release(Q.lbase, vg);
     Q.state = 3;
+    m->free_ids.push_back(q);
     return B40C_OK;
 }
 
@@ -1264,6 +1280,7 @@ void b40c_msbfs_destroy(b40c_msbfs* m) {
     if (m->h_ctr) (void)hipHostFree(m->h_ctr);
     if (m->h_newbits) (void)hipHostFree(m->h_newbits);
     if (m->stream) (void)hipStreamDestroy(m->stream);
+    b40c_multigraph_destroy(m->mg);
     delete m;
 }
 

@@ -241,7 +241,12 @@ typedef struct b40c_msbfs b40c_msbfs;
 b40c_status b40c_multigraph_pack(b40c_graph* const* graphs, int32_t ngraphs, b40c_multigraph** out);
 /* vbase: ngraphs + 1 entries, ebase: ngraphs + 1 entries (either nullable). */
 b40c_status b40c_multigraph_info(const b40c_multigraph* mg, int32_t* ngraphs, int32_t* vbase, int64_t* ebase);
+/* Drops the caller's reference; engines created on the pack keep their own,
+ * so the pack may be destroyed before them. */
 void b40c_multigraph_destroy(b40c_multigraph* mg);
+/* label_budget_bytes: device memory for per-query depth + pred slots (8 B per
+ * vertex of the query's graph).  max_queries: queries submitted and not yet
+ * collected by b40c_msbfs_result (ids are reused after collection). */
 b40c_status b40c_msbfs_create(b40c_multigraph* mg, int64_t label_budget_bytes, int32_t max_queries,
                               b40c_msbfs** out);
 /* Direction switch parameters and workgroups per level launch (0 keeps). */
@@ -262,7 +267,7 @@ b40c_status b40c_msbfs_run(b40c_msbfs* m);
 /* Finished queries not reported before (up to max_ids). */
 b40c_status b40c_msbfs_poll(b40c_msbfs* m, int32_t* ids, int32_t max_ids, int32_t* count);
 /* Labels of a finished query (nullable outputs) and its search depth; frees
- * its label slot. */
+ * its label slot, and its id may be returned by a later submission. */
 b40c_status b40c_msbfs_result(b40c_msbfs* m, int32_t query_id, int32_t* depth, int32_t* pred,
                               int32_t* search_depth);
 void* b40c_msbfs_stream(b40c_msbfs* m);

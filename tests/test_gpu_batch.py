@@ -90,24 +90,31 @@ def test_continuous_batching_and_slot_reuse(mods, orc, pool):
     waves = [[(g_big, s) for s in sources(orc, host, g_big, 5, 31)],
              [(1, s) for s in sources(orc, host, 1, 70, 32)],
              [(3, s) for s in sources(orc, host, 3, 6, 33)] + [(5, s) for s in sources(orc, host, 5, 3, 34)]]
+    # Ids are reused once a result is collected, so each id maps to its
+    # query only until then.
     ids = {}
-    finished = {}
+    finished = []
+    submitted = 0
+
+    def collect():
+        for q in eng.poll():
+            g, s = ids.pop(q)
+            finished.append(((g, s), eng.result(q, g)))
+
     for wave in waves:
         for g, s in wave:
             ids[eng.submit(g, s)] = (g, s)
+            submitted += 1
         for _ in range(2):
             eng.step()
-        for q in eng.poll():
-            finished[q] = eng.result(q, ids[q][0])
+        collect()
     while True:
         r, w = eng.step()
-        for q in eng.poll():
-            finished[q] = eng.result(q, ids[q][0])
+        collect()
         if r == 0 and w == 0:
             break
-    assert sorted(finished) == sorted(ids)
-    for q, (depth, pred, sd) in finished.items():
-        g, s = ids[q]
+    assert len(finished) == submitted and not ids
+    for (g, s), (depth, pred, sd) in finished:
         check_query(orc, host, g, s, depth, pred, sd)
 
 
@@ -179,21 +186,62 @@ def test_staggered_sessions_with_idle_gaps(mods, orc, gap):
     eng = batch.MultiSourceBFS(mg, label_budget_bytes=1 << 28)
     eng.set_admission(gap)
     rng = np.random.default_rng(70 + gap)
-    ids, out = {}, {}
+    ids, out = {}, []
+    submitted = 0
+
+    def collect():
+        for q in eng.poll():
+            g, s = ids.pop(q)  # the id may be reused by the next submission
+            out.append(((g, s), eng.result(q, g)))
+
     for session in range(3):
         arrivals = [int(rng.integers(0, len(specs))) for _ in range(40)]
         for k, g in enumerate(arrivals):
             s = sources(orc, host, g, 1, 1000 * session + k)[0]
             ids[eng.submit(g, s)] = (g, s)
+            submitted += 1
             eng.step()
-            for q in eng.poll():
-                out[q] = eng.result(q, ids[q][0])
+            collect()
         eng.run()  # drain: the engine goes idle before the next session
-        for q in eng.poll():
-            out[q] = eng.result(q, ids[q][0])
-    assert sorted(out) == sorted(ids)
-    for q, (depth, pred, sd) in out.items():
-        g, s = ids[q]
+        collect()
+    assert len(out) == submitted and not ids
+    for (g, s), (depth, pred, sd) in out:
         check_query(orc, host, g, s, depth, pred, sd)
     eng.close()
     mg.close()
+
+
+def test_query_ids_recycled_and_pack_released_first(mods, orc):
+    """max_queries bounds the queries submitted and not yet collected: ids
+    come back after their results are taken, so a long-running engine keeps
+    serving.  The engine holds its own reference to the multigraph, which
+    may be destroyed first."""
+    bfs, batch = mods
+    from back40computing_amd import B40CError, _lib
+    graphs = [bfs.Graph.rmat(9, 16, 0.57, 0.19, 0.19, 81), bfs.Graph.lattice3d(8, 82)]
+    host = [g.to_host() for g in graphs]
+    mg = batch.MultiGraph(graphs)
+    eng = batch.MultiSourceBFS(mg, label_budget_bytes=1 << 24, max_queries=8)
+    nodes = [mg.nodes(0), mg.nodes(1)]
+    mg.close()  # the engine keeps the pack alive
+    seen = set()
+    for rnd in range(4):
+        batch_q = [(k % 2, int(orc.pick_sources(host[k % 2][0], 100 * rnd + k, 1)[0])) for k in range(8)]
+        ids = {}
+        for g, s in batch_q:
+            ids[eng.submit(g, s)] = (g, s)
+        assert len(ids) == 8
+        seen |= set(ids)
+        with pytest.raises(B40CError) as ei:
+            eng.submit(0, 0)
+        assert ei.value.status == _lib.ERR_ADMISSION
+        eng.run()
+        for q in eng.poll():
+            g, s = ids.pop(q)
+            depth, pred, sd = eng.result(q, g)  # sizes come from the Python-side copy of the offsets
+            assert depth.size == nodes[g]
+            check_query(orc, host, g, s, depth, pred, sd)
+        assert not ids
+    assert seen == set(range(8))
+    eng.close()
+

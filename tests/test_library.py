@@ -56,6 +56,19 @@ def test_host_argument_validation(L):
     c2 = np.array([1, 0], np.int32)
     st = L.b40c_graph_from_host_csr(2, 2, _lib.ptr(off), _lib.ptr(c2), ctypes.byref(out))
     assert st == _lib.ERR_INVALID_ARG
+    # CSR offsets that decrease, and a column id outside [0, n), are refused
+    # on the host (the kernels use both as addresses).
+    off = np.array([0, 2, 1, 3], np.int32)
+    c3 = np.array([1, 2, 0], np.int32)
+    st = L.b40c_graph_from_host_csr(3, 3, _lib.ptr(off), _lib.ptr(c3), ctypes.byref(out))
+    assert st == _lib.ERR_INVALID_ARG
+    assert "decrease" in _lib.last_error()
+    off = np.array([0, 1, 2, 3], np.int32)
+    for bad in (3, -1, 1 << 30):
+        c4 = np.array([1, bad, 0], np.int32)
+        st = L.b40c_graph_from_host_csr(3, 3, _lib.ptr(off), _lib.ptr(c4), ctypes.byref(out))
+        assert st == _lib.ERR_INVALID_ARG
+        assert "out of range" in _lib.last_error()
     # Null selector for the radix sort.
     assert L.b40c_radix_sort_u64(None, None, 10, 0, 64, None) == _lib.ERR_INVALID_ARG
     # Unknown strategy and null handles.
