@@ -136,6 +136,10 @@ SIGNATURES = {
     "b40c_graph_write_metis": (_i32, [_vp, ctypes.c_char_p]),
     "b40c_radix_sort_u64": (_i32, [_vp, _vp, _i64, _i32, _i32, _p_i32]),
     "b40c_radix_sort_pairs_u32": (_i32, [_vp, _vp, _vp, _vp, _i64, _i32, _i32, _p_i32]),
+    "b40c_radix_sort": (_i32, [_i32, _vp, _vp, _vp, _vp, _i64, _i32, _i32, _p_i32]),
+    "b40c_sort_enactor_create": (_i32, [_pp]),
+    "b40c_sort_enactor_sort": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, _i64, _i32, _i32, _p_i32]),
+    "b40c_sort_enactor_destroy": (None, [_vp]),
     "b40c_device_alloc": (_i32, [_pp, _i64]),
     "b40c_device_free": (_i32, [_vp]),
     "b40c_memcpy_h2d": (_i32, [_vp, _vp, _i64]),

@@ -38,7 +38,23 @@ struct DeviceBuffer {
     T* as() const { return static_cast<T*>(ptr); }
 };
 
-// radix_sort.hip
+// radix_sort.hip.  A sort's device scratch: the spine (256 counters per
+// workgroup), two digit-total buffers, the device-side selector and the
+// trivial-pass flag, carved out of one block of `words` uint32 words.
+struct SortScratch {
+    uint32_t* spine = nullptr;
+    uint32_t* digit_total = nullptr;
+    int32_t* sel = nullptr;
+    uint32_t* trivial = nullptr;
+    size_t words = 0;
+};
+// Words of scratch a sort of n keys needs on a device with num_cus CUs.
+size_t sort_scratch_words(int64_t n, int num_cus);
+SortScratch sort_scratch_view(uint32_t* words, size_t count);
+// Launches the passes (no allocation, no synchronization); the selector is
+// left in *sc.sel.  `launched` is false when there was nothing to sort.
+b40c_status radix_sort_async(int32_t key_type, void* k0, void* k1, uint32_t* v0, uint32_t* v1, int64_t n,
+                             int begin_bit, int end_bit, const SortScratch& sc, hipStream_t stream, bool& launched);
 b40c_status radix_sort_u64(uint64_t* k0, uint64_t* k1, int64_t n, int begin_bit, int end_bit, int32_t* selector,
                            hipStream_t stream);
 b40c_status radix_sort_pairs_u32(uint32_t* k0, uint32_t* k1, uint32_t* v0, uint32_t* v1, int64_t n, int begin_bit,

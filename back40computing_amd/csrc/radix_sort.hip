@@ -15,11 +15,23 @@
 //    tile, so ranks are stable.  The tile is exchanged through LDS into digit
 //    order and written out as contiguous per-digit runs (two-phase scatter,
 //    partition/downsweep/tile.cuh:779-1030).
+//  * Signed and floating-point keys (KeyTraits, radix_sort/sort_utils.cuh:
+//    87-183) are ordered by the reference's bit twiddles, applied in
+//    registers whenever a digit is extracted: memory always holds the
+//    caller's keys, so no pass rewrites them and the trivial-pass skip stays
+//    valid for every key type.
 //  * The trivial-pass early exit (downsweep/kernel.cuh DownsweepPass) is
 //    decided on the device: the spine kernel flags a pass whose keys all share
 //    one digit, the downsweep then does nothing and a device-side selector
-//    (DoubleBuffer::selector) is left unflipped.  The host reads the selector
-//    once at the end.
+//    (DoubleBuffer::selector) is left unflipped.
+//  * Scratch (spine, digit totals, selector, trivial flag) comes from the
+//    caller as a SortScratch view: a per-call allocation for the stateless
+//    entry points, the sort enactor's own buffer, or a fixed device array for
+//    the allocation-free entry point.  Launching a sort allocates nothing and
+//    does not synchronize; only reading the selector back does.
+#include <mutex>
+#include <vector>
+
 #include "common.hpp"
 #include "internal.hpp"
 
@@ -31,27 +43,51 @@ constexpr int RS_WAVES = RS_THREADS / kWave;
 constexpr int RS_ITEMS = 8;
 constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
 constexpr int RS_BINS = 256;
+constexpr int RS_WGS_PER_CU = 4;
 
-template <typename K>
+// Order-preserving views of the key bits (sort_utils.cuh:87-183):
+// CONV 0 unsigned (as is), 1 signed (flip the sign bit), 2 IEEE float
+// (negative: flip all bits; positive: flip the sign bit).
+enum KeyConv { KC_UNSIGNED = 0, KC_SIGNED = 1, KC_FLOAT = 2 };
+
+template <typename K, int CONV>
+__device__ __forceinline__ K ordered_bits(K k) {
+    constexpr K HIGH = (K)1 << (8 * sizeof(K) - 1);
+    if (CONV == KC_SIGNED) return k ^ HIGH;
+    if (CONV == KC_FLOAT) return k ^ ((k & HIGH) ? (K)~(K)0 : HIGH);
+    return k;
+}
+
+template <typename K, int CONV>
 __device__ __forceinline__ uint32_t digit_of(K key, int shift, uint32_t mask) {
-    return (uint32_t)(key >> shift) & mask;
+    return (uint32_t)(ordered_bits<K, CONV>(key) >> shift) & mask;
+}
+
+// Start of a sort: digit totals of the first pass, selector and trivial flag
+// cleared (kernels, not memsets, so a caller may capture the launches).
+__global__ void k_sort_init(uint32_t* digit_total0, int32_t* sel, uint32_t* trivial) {
+    digit_total0[threadIdx.x] = 0;
+    if (threadIdx.x == 0) {
+        *sel = 0;
+        *trivial = 0;
+    }
 }
 
 // Per-workgroup digit histogram of its tile range; counts go to the spine
 // (digit-major) and to the global digit totals.
-template <typename K>
-__global__ __launch_bounds__(RS_THREADS) void k_upsweep(K* const* bufs, const int32_t* sel, int64_t n, int shift,
-                                                        uint32_t mask, int64_t tiles_per_wg, uint32_t* spine,
-                                                        uint32_t* digit_total, int G) {
+template <typename K, int CONV>
+__global__ __launch_bounds__(RS_THREADS) void k_upsweep(const K* k0, const K* k1, const int32_t* sel, int64_t n,
+                                                        int shift, uint32_t mask, int64_t tiles_per_wg,
+                                                        uint32_t* spine, uint32_t* digit_total, int G) {
     __shared__ uint32_t hist[RS_WAVES][RS_BINS];
     const int tid = threadIdx.x, wid = tid / kWave;
     for (int i = tid; i < RS_WAVES * RS_BINS; i += RS_THREADS) (&hist[0][0])[i] = 0;
     __syncthreads();
-    const K* keys = bufs[*sel];
+    const K* keys = *sel ? k1 : k0;
     const int64_t begin = (int64_t)blockIdx.x * tiles_per_wg * RS_TILE;
     const int64_t end = min(n, begin + tiles_per_wg * RS_TILE);
     for (int64_t i = begin + tid; i < end; i += RS_THREADS) {
-        uint32_t d = digit_of(keys[i], shift, mask);
+        uint32_t d = digit_of<K, CONV>(keys[i], shift, mask);
         atomicAdd(&hist[wid][d], 1u);
     }
     __syncthreads();
@@ -63,12 +99,15 @@ __global__ __launch_bounds__(RS_THREADS) void k_upsweep(K* const* bufs, const in
 }
 
 // Exclusive scan of the spine, one workgroup per digit.  Digit d's base is the
-// sum of the totals of the smaller digits.  Flags a trivial pass.
-__global__ __launch_bounds__(RS_THREADS) void k_spine(uint32_t* spine, const uint32_t* digit_total, int G, int64_t n,
+// sum of the totals of the smaller digits.  Flags a trivial pass, and
+// workgroup 0 clears the other digit-total buffer for the next pass.
+__global__ __launch_bounds__(RS_THREADS) void k_spine(uint32_t* spine, const uint32_t* digit_total,
+                                                      uint32_t* next_digit_total, int G, int64_t n,
                                                       uint32_t* trivial) {
     __shared__ int s_wave[RS_WAVES];
     const int d = blockIdx.x, tid = threadIdx.x;
     const uint32_t t = digit_total[tid];
+    if (d == 0) next_digit_total[tid] = 0;
     int total_below;
     int contrib = (tid < d) ? (int)t : 0;
     (void)wg_exclusive_scan<RS_THREADS>(contrib, total_below, s_wave);
@@ -85,10 +124,11 @@ __global__ __launch_bounds__(RS_THREADS) void k_spine(uint32_t* spine, const uin
     }
 }
 
-template <typename K, bool VALS>
-__global__ __launch_bounds__(RS_THREADS) void k_downsweep(K* const* kbufs, uint32_t* const* vbufs, const int32_t* sel,
-                                                          int64_t n, int shift, uint32_t mask, int64_t tiles_per_wg,
-                                                          const uint32_t* spine, int G, const uint32_t* trivial) {
+template <typename K, int CONV, bool VALS>
+__global__ __launch_bounds__(RS_THREADS) void k_downsweep(K* k0, K* k1, uint32_t* v0, uint32_t* v1,
+                                                          const int32_t* sel, int64_t n, int shift, uint32_t mask,
+                                                          int64_t tiles_per_wg, const uint32_t* spine, int G,
+                                                          const uint32_t* trivial) {
     if (*trivial) return;
     __shared__ K stage_k[RS_TILE];
     __shared__ uint32_t stage_v[VALS ? RS_TILE : 1];
@@ -99,10 +139,10 @@ __global__ __launch_bounds__(RS_THREADS) void k_downsweep(K* const* kbufs, uint3
 
     const int tid = threadIdx.x, lane = lane_id(), wid = tid / kWave;
     const int s = *sel;
-    const K* kin = kbufs[s];
-    K* kout = kbufs[s ^ 1];
-    const uint32_t* vin = VALS ? vbufs[s] : nullptr;
-    uint32_t* vout = VALS ? vbufs[s ^ 1] : nullptr;
+    const K* kin = s ? k1 : k0;
+    K* kout = s ? k0 : k1;
+    const uint32_t* vin = VALS ? (s ? v1 : v0) : nullptr;
+    uint32_t* vout = VALS ? (s ? v0 : v1) : nullptr;
 
     carry[tid] = spine[(int64_t)tid * G + blockIdx.x];
     const int64_t begin = (int64_t)blockIdx.x * tiles_per_wg * RS_TILE;
@@ -124,7 +164,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_downsweep(K* const* kbufs, uint3
             const bool valid = idx < tile_n;
             key[j] = valid ? kin[tile_base + idx] : (K)0;
             if (VALS) val[j] = valid ? vin[tile_base + idx] : 0u;
-            const uint32_t d = digit_of(key[j], shift, mask);
+            const uint32_t d = digit_of<K, CONV>(key[j], shift, mask);
             dig[j] = d;
             uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -163,7 +203,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_downsweep(K* const* kbufs, uint3
         __syncthreads();
         for (int k = tid; k < tile_n; k += RS_THREADS) {
             const K kk = stage_k[k];
-            const uint32_t d = digit_of(kk, shift, mask);
+            const uint32_t d = digit_of<K, CONV>(kk, shift, mask);
             const uint32_t out = carry[d] + (uint32_t)k - tile_excl[d];
             kout[out] = kk;
             if (VALS) vout[out] = stage_v[k];
@@ -179,86 +219,233 @@ __global__ void k_flip_selector(int32_t* sel, uint32_t* trivial) {
     *trivial = 0;
 }
 
-struct SortScratch {
-    DeviceBuffer spine, digit_total, flags, bufptrs;
+// Launch shape for n keys on a device with `num_cus` CUs: at most
+// RS_WGS_PER_CU workgroups per CU, each owning a whole number of tiles.
+struct SortPlan {
+    int64_t tiles_per_wg = 1;
+    int G = 1;
 };
 
-template <typename K, bool VALS>
-b40c_status radix_sort_impl(K* k0, K* k1, uint32_t* v0, uint32_t* v1, int64_t n, int begin_bit, int end_bit,
-                            int32_t* selector, hipStream_t stream) {
-    if (n < 0 || begin_bit < 0 || end_bit > (int)(8 * sizeof(K)) || begin_bit > end_bit)
-        return fail(B40C_ERR_INVALID_ARG, "radix sort: bad arguments (n=%lld bits=[%d,%d))", (long long)n,
-                    begin_bit, end_bit);
-    if (n > 0xFFFFFFFFll) return fail(B40C_ERR_UNSUPPORTED, "radix sort: n=%lld exceeds 2^32-1", (long long)n);
-    *selector = 0;
-    if (n <= 1 || begin_bit == end_bit) return B40C_OK;
+SortPlan sort_plan(int64_t n, int num_cus) {
+    SortPlan p;
+    const int64_t num_tiles = std::max<int64_t>(1, (n + RS_TILE - 1) / RS_TILE);
+    const int64_t max_grid = (int64_t)num_cus * RS_WGS_PER_CU;
+    p.tiles_per_wg = (num_tiles + max_grid - 1) / max_grid;
+    p.G = (int)((num_tiles + p.tiles_per_wg - 1) / p.tiles_per_wg);
+    return p;
+}
 
-    DeviceInfo info;
-    B40C_TRY(device_info(info));
-    const int64_t num_tiles = (n + RS_TILE - 1) / RS_TILE;
-    const int64_t max_grid = (int64_t)info.num_cus * 4;
-    const int64_t tiles_per_wg = (num_tiles + max_grid - 1) / max_grid;
-    const int G = (int)((num_tiles + tiles_per_wg - 1) / tiles_per_wg);
-
-    SortScratch sc;
-    B40C_TRY(sc.spine.alloc((size_t)RS_BINS * G * sizeof(uint32_t)));
-    B40C_TRY(sc.digit_total.alloc(RS_BINS * sizeof(uint32_t)));
-    B40C_TRY(sc.flags.alloc(2 * sizeof(uint32_t)));
-    B40C_TRY(sc.bufptrs.alloc(4 * sizeof(void*)));
-    void* host_ptrs[4] = {k0, k1, v0, v1};
-    B40C_HIP(hipMemcpyAsync(sc.bufptrs.ptr, host_ptrs, sizeof(host_ptrs), hipMemcpyHostToDevice, stream));
-    B40C_HIP(hipMemsetAsync(sc.flags.ptr, 0, 2 * sizeof(uint32_t), stream));
-    K* const* kbufs = (K* const*)sc.bufptrs.ptr;
-    uint32_t* const* vbufs = (uint32_t* const*)((void**)sc.bufptrs.ptr + 2);
-    int32_t* d_sel = (int32_t*)sc.flags.ptr;
-    uint32_t* d_trivial = (uint32_t*)sc.flags.ptr + 1;
-    uint32_t* spine = (uint32_t*)sc.spine.ptr;
-    uint32_t* dtot = (uint32_t*)sc.digit_total.ptr;
-
-    for (int shift = begin_bit; shift < end_bit; shift += 8) {
-        const int bits = min(8, end_bit - shift);
+template <typename K, int CONV, bool VALS>
+void launch_passes(K* k0, K* k1, uint32_t* v0, uint32_t* v1, int64_t n, int begin_bit, int end_bit,
+                   const SortPlan& plan, const SortScratch& sc, hipStream_t stream) {
+    uint32_t* dtot[2] = {sc.digit_total, sc.digit_total + RS_BINS};
+    hipLaunchKernelGGL(k_sort_init, dim3(1), dim3(RS_BINS), 0, stream, dtot[0], sc.sel, sc.trivial);
+    int pass = 0;
+    for (int shift = begin_bit; shift < end_bit; shift += 8, ++pass) {
+        const int bits = std::min(8, end_bit - shift);
         const uint32_t mask = (1u << bits) - 1u;
-        B40C_HIP(hipMemsetAsync(dtot, 0, RS_BINS * sizeof(uint32_t), stream));
-        hipLaunchKernelGGL(k_upsweep<K>, dim3(G), dim3(RS_THREADS), 0, stream, kbufs, d_sel, n, shift, mask,
-                           tiles_per_wg, spine, dtot, G);
-        hipLaunchKernelGGL(k_spine, dim3(RS_BINS), dim3(RS_THREADS), 0, stream, spine, dtot, G, n, d_trivial);
-        hipLaunchKernelGGL((k_downsweep<K, VALS>), dim3(G), dim3(RS_THREADS), 0, stream, kbufs, vbufs, d_sel, n,
-                           shift, mask, tiles_per_wg, spine, G, d_trivial);
-        hipLaunchKernelGGL(k_flip_selector, dim3(1), dim3(1), 0, stream, d_sel, d_trivial);
+        uint32_t* cur = dtot[pass & 1];
+        uint32_t* next = dtot[(pass + 1) & 1];
+        hipLaunchKernelGGL((k_upsweep<K, CONV>), dim3(plan.G), dim3(RS_THREADS), 0, stream, k0, k1, sc.sel, n, shift,
+                           mask, plan.tiles_per_wg, sc.spine, cur, plan.G);
+        hipLaunchKernelGGL(k_spine, dim3(RS_BINS), dim3(RS_THREADS), 0, stream, sc.spine, cur, next, plan.G, n,
+                           sc.trivial);
+        hipLaunchKernelGGL((k_downsweep<K, CONV, VALS>), dim3(plan.G), dim3(RS_THREADS), 0, stream, k0, k1, v0, v1,
+                           sc.sel, n, shift, mask, plan.tiles_per_wg, sc.spine, plan.G, sc.trivial);
+        hipLaunchKernelGGL(k_flip_selector, dim3(1), dim3(1), 0, stream, sc.sel, sc.trivial);
     }
-    B40C_HIP(hipGetLastError());
-    int32_t h_sel = 0;
-    B40C_HIP(hipMemcpyAsync(&h_sel, d_sel, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
-    B40C_HIP(hipStreamSynchronize(stream));
-    *selector = h_sel;
+}
+
+template <typename K, int CONV>
+void launch_typed(void* k0, void* k1, uint32_t* v0, uint32_t* v1, int64_t n, int begin_bit, int end_bit,
+                  const SortPlan& plan, const SortScratch& sc, hipStream_t stream) {
+    if (v0)
+        launch_passes<K, CONV, true>((K*)k0, (K*)k1, v0, v1, n, begin_bit, end_bit, plan, sc, stream);
+    else
+        launch_passes<K, CONV, false>((K*)k0, (K*)k1, nullptr, nullptr, n, begin_bit, end_bit, plan, sc, stream);
+}
+
+int key_bits(int32_t key_type) {
+    switch (key_type) {
+        case B40C_KEY_U32:
+        case B40C_KEY_I32:
+        case B40C_KEY_F32:
+            return 32;
+        case B40C_KEY_U64:
+        case B40C_KEY_I64:
+        case B40C_KEY_F64:
+            return 64;
+    }
+    return 0;
+}
+
+b40c_status validate(const char* who, int32_t key_type, const void* k0, const void* k1, const uint32_t* v0,
+                     const uint32_t* v1, int64_t n, int begin_bit, int end_bit, const int32_t* selector) {
+    const int kb = key_bits(key_type);
+    if (!kb) return fail(B40C_ERR_INVALID_ARG, "%s: unknown key type %d", who, key_type);
+    if (!selector || (n > 0 && (!k0 || !k1)) || ((v0 == nullptr) != (v1 == nullptr)))
+        return fail(B40C_ERR_INVALID_ARG, "%s: null pointer", who);
+    if (n < 0 || begin_bit < 0 || end_bit > kb || begin_bit > end_bit)
+        return fail(B40C_ERR_INVALID_ARG, "%s: bad arguments (n=%lld bits=[%d,%d))", who, (long long)n, begin_bit,
+                    end_bit);
+    if (n > 0xFFFFFFFFll) return fail(B40C_ERR_UNSUPPORTED, "%s: n=%lld exceeds 2^32-1", who, (long long)n);
     return B40C_OK;
 }
 
 }  // namespace
 
+size_t sort_scratch_words(int64_t n, int num_cus) {
+    const SortPlan p = sort_plan(n, num_cus);
+    return (size_t)RS_BINS * (size_t)p.G + 2 * RS_BINS + 2;
+}
+
+SortScratch sort_scratch_view(uint32_t* words, size_t count) {
+    SortScratch sc;
+    sc.words = count;
+    sc.digit_total = words;
+    sc.sel = (int32_t*)(words + 2 * RS_BINS);
+    sc.trivial = words + 2 * RS_BINS + 1;
+    sc.spine = words + 2 * RS_BINS + 2;
+    return sc;
+}
+
+b40c_status radix_sort_async(int32_t key_type, void* k0, void* k1, uint32_t* v0, uint32_t* v1, int64_t n,
+                             int begin_bit, int end_bit, const SortScratch& sc, hipStream_t stream, bool& launched) {
+    launched = false;
+    if (n <= 1 || begin_bit == end_bit) return B40C_OK;
+    DeviceInfo info;
+    B40C_TRY(device_info(info));
+    const SortPlan plan = sort_plan(n, info.num_cus);
+    if (sort_scratch_words(n, info.num_cus) > sc.words)
+        return fail(B40C_ERR_INVALID_ARG, "radix sort: scratch of %zu words is smaller than the %zu needed", sc.words,
+                    sort_scratch_words(n, info.num_cus));
+    switch (key_type) {
+        case B40C_KEY_U32: launch_typed<uint32_t, KC_UNSIGNED>(k0, k1, v0, v1, n, begin_bit, end_bit, plan, sc, stream); break;
+        case B40C_KEY_I32: launch_typed<uint32_t, KC_SIGNED>(k0, k1, v0, v1, n, begin_bit, end_bit, plan, sc, stream); break;
+        case B40C_KEY_F32: launch_typed<uint32_t, KC_FLOAT>(k0, k1, v0, v1, n, begin_bit, end_bit, plan, sc, stream); break;
+        case B40C_KEY_U64: launch_typed<uint64_t, KC_UNSIGNED>(k0, k1, v0, v1, n, begin_bit, end_bit, plan, sc, stream); break;
+        case B40C_KEY_I64: launch_typed<uint64_t, KC_SIGNED>(k0, k1, v0, v1, n, begin_bit, end_bit, plan, sc, stream); break;
+        case B40C_KEY_F64: launch_typed<uint64_t, KC_FLOAT>(k0, k1, v0, v1, n, begin_bit, end_bit, plan, sc, stream); break;
+        default: return fail(B40C_ERR_INVALID_ARG, "radix sort: unknown key type %d", key_type);
+    }
+    B40C_HIP(hipGetLastError());
+    launched = true;
+    return B40C_OK;
+}
+
+// Runs a sort with its scratch and reads the selector back (one stream
+// synchronization).
+b40c_status radix_sort_sync(int32_t key_type, void* k0, void* k1, uint32_t* v0, uint32_t* v1, int64_t n, int begin_bit,
+                            int end_bit, const SortScratch& sc, int32_t* selector, hipStream_t stream) {
+    bool launched = false;
+    B40C_TRY(radix_sort_async(key_type, k0, k1, v0, v1, n, begin_bit, end_bit, sc, stream, launched));
+    int32_t h_sel = 0;
+    if (launched) {
+        B40C_HIP(hipMemcpyAsync(&h_sel, sc.sel, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+        B40C_HIP(hipStreamSynchronize(stream));
+    }
+    *selector = h_sel;
+    return B40C_OK;
+}
+
+// Stateless form: scratch allocated for this call.
+b40c_status radix_sort_once(int32_t key_type, void* k0, void* k1, uint32_t* v0, uint32_t* v1, int64_t n, int begin_bit,
+                            int end_bit, int32_t* selector, hipStream_t stream) {
+    *selector = 0;
+    if (n <= 1 || begin_bit == end_bit) return B40C_OK;
+    DeviceInfo info;
+    B40C_TRY(device_info(info));
+    DeviceBuffer mem;
+    const size_t words = sort_scratch_words(n, info.num_cus);
+    B40C_TRY(mem.alloc(words * sizeof(uint32_t)));
+    return radix_sort_sync(key_type, k0, k1, v0, v1, n, begin_bit, end_bit, sort_scratch_view(mem.as<uint32_t>(), words),
+                           selector, stream);
+}
+
 b40c_status radix_sort_u64(uint64_t* k0, uint64_t* k1, int64_t n, int begin_bit, int end_bit, int32_t* selector,
                            hipStream_t stream) {
-    return radix_sort_impl<uint64_t, false>(k0, k1, nullptr, nullptr, n, begin_bit, end_bit, selector, stream);
+    B40C_TRY(validate("radix sort", B40C_KEY_U64, k0, k1, nullptr, nullptr, n, begin_bit, end_bit, selector));
+    return radix_sort_once(B40C_KEY_U64, k0, k1, nullptr, nullptr, n, begin_bit, end_bit, selector, stream);
 }
 
 b40c_status radix_sort_pairs_u32(uint32_t* k0, uint32_t* k1, uint32_t* v0, uint32_t* v1, int64_t n, int begin_bit,
                                  int end_bit, int32_t* selector, hipStream_t stream) {
-    return radix_sort_impl<uint32_t, true>(k0, k1, v0, v1, n, begin_bit, end_bit, selector, stream);
+    B40C_TRY(validate("radix sort", B40C_KEY_U32, k0, k1, v0, v1, n, begin_bit, end_bit, selector));
+    return radix_sort_once(B40C_KEY_U32, k0, k1, v0, v1, n, begin_bit, end_bit, selector, stream);
 }
 
 }  // namespace b40c
 
-extern "C" b40c_status b40c_radix_sort_u64(uint64_t* d_keys0, uint64_t* d_keys1, int64_t n, int32_t begin_bit,
-                                           int32_t end_bit, int32_t* selector) {
-    if (!selector || (n > 0 && (!d_keys0 || !d_keys1)))
-        return b40c::fail(B40C_ERR_INVALID_ARG, "b40c_radix_sort_u64: null pointer");
-    return b40c::radix_sort_u64(d_keys0, d_keys1, n, begin_bit, end_bit, selector, nullptr);
+// Sort enactor: owns its scratch across calls, like the reference Enactor's
+// spine storage (radix_sort/enactor.cuh:96-140).  One stream per enactor.
+struct b40c_sort_enactor {
+    b40c::DeviceBuffer scratch;
+    hipStream_t stream = nullptr;
+};
+
+using namespace b40c;
+
+extern "C" {
+
+b40c_status b40c_radix_sort_u64(uint64_t* d_keys0, uint64_t* d_keys1, int64_t n, int32_t begin_bit, int32_t end_bit,
+                                int32_t* selector) {
+    B40C_TRY(validate("b40c_radix_sort_u64", B40C_KEY_U64, d_keys0, d_keys1, nullptr, nullptr, n, begin_bit, end_bit,
+                      selector));
+    return radix_sort_once(B40C_KEY_U64, d_keys0, d_keys1, nullptr, nullptr, n, begin_bit, end_bit, selector, nullptr);
 }
 
-extern "C" b40c_status b40c_radix_sort_pairs_u32(uint32_t* d_keys0, uint32_t* d_keys1, uint32_t* d_vals0,
-                                                 uint32_t* d_vals1, int64_t n, int32_t begin_bit, int32_t end_bit,
-                                                 int32_t* selector) {
-    if (!selector || (n > 0 && (!d_keys0 || !d_keys1 || !d_vals0 || !d_vals1)))
-        return b40c::fail(B40C_ERR_INVALID_ARG, "b40c_radix_sort_pairs_u32: null pointer");
-    return b40c::radix_sort_pairs_u32(d_keys0, d_keys1, d_vals0, d_vals1, n, begin_bit, end_bit, selector, nullptr);
+b40c_status b40c_radix_sort_pairs_u32(uint32_t* d_keys0, uint32_t* d_keys1, uint32_t* d_vals0, uint32_t* d_vals1,
+                                      int64_t n, int32_t begin_bit, int32_t end_bit, int32_t* selector) {
+    if (n > 0 && (!d_vals0 || !d_vals1))
+        return fail(B40C_ERR_INVALID_ARG, "b40c_radix_sort_pairs_u32: null pointer");
+    B40C_TRY(validate("b40c_radix_sort_pairs_u32", B40C_KEY_U32, d_keys0, d_keys1, d_vals0, d_vals1, n, begin_bit,
+                      end_bit, selector));
+    return radix_sort_once(B40C_KEY_U32, d_keys0, d_keys1, d_vals0, d_vals1, n, begin_bit, end_bit, selector, nullptr);
 }
+
+b40c_status b40c_radix_sort(int32_t key_type, void* d_keys0, void* d_keys1, uint32_t* d_vals0, uint32_t* d_vals1,
+                            int64_t n, int32_t begin_bit, int32_t end_bit, int32_t* selector) {
+    B40C_TRY(validate("b40c_radix_sort", key_type, d_keys0, d_keys1, d_vals0, d_vals1, n, begin_bit, end_bit, selector));
+    return radix_sort_once(key_type, d_keys0, d_keys1, d_vals0, d_vals1, n, begin_bit, end_bit, selector, nullptr);
+}
+
+b40c_status b40c_sort_enactor_create(b40c_sort_enactor** out) {
+    if (!out) return fail(B40C_ERR_INVALID_ARG, "b40c_sort_enactor_create: null out");
+    b40c_sort_enactor* e = new b40c_sort_enactor();
+    const hipError_t err = hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking);
+    if (err != hipSuccess) {
+        delete e;
+        return fail(B40C_ERR_HIP, "b40c_sort_enactor_create: %s", hipGetErrorString(err));
+    }
+    *out = e;
+    return B40C_OK;
+}
+
+b40c_status b40c_sort_enactor_sort(b40c_sort_enactor* e, int32_t key_type, void* d_keys0, void* d_keys1,
+                                   uint32_t* d_vals0, uint32_t* d_vals1, int64_t n, int32_t begin_bit, int32_t end_bit,
+                                   int32_t* selector) {
+    if (!e) return fail(B40C_ERR_INVALID_ARG, "b40c_sort_enactor_sort: null enactor");
+    B40C_TRY(validate("b40c_sort_enactor_sort", key_type, d_keys0, d_keys1, d_vals0, d_vals1, n, begin_bit, end_bit,
+                      selector));
+    *selector = 0;
+    if (n <= 1 || begin_bit == end_bit) return B40C_OK;
+    DeviceInfo info;
+    B40C_TRY(device_info(info));
+    const size_t words = sort_scratch_words(n, info.num_cus);
+    // Grow-only: later sorts of at most this size allocate nothing.
+    B40C_TRY(e->scratch.reserve(words * sizeof(uint32_t)));
+    return radix_sort_sync(key_type, d_keys0, d_keys1, d_vals0, d_vals1, n, begin_bit, end_bit,
+                           sort_scratch_view(e->scratch.as<uint32_t>(), e->scratch.bytes / sizeof(uint32_t)), selector,
+                           e->stream);
+}
+
+void b40c_sort_enactor_destroy(b40c_sort_enactor* e) {
+    if (!e) return;
+    if (e->stream) {
+        (void)hipStreamSynchronize(e->stream);
+        (void)hipStreamDestroy(e->stream);
+    }
+    delete e;
+}
+
+}  // extern "C"

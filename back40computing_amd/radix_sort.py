@@ -51,26 +51,64 @@ class DoubleBuffer:
         self.selector = 0
 
 
+KEY_TYPES = {
+    np.dtype(np.uint32): 0, np.dtype(np.int32): 1, np.dtype(np.float32): 2,
+    np.dtype(np.uint64): 3, np.dtype(np.int64): 4, np.dtype(np.float64): 5,
+}
+
+
 class Enactor:
-    """radix_sort::Enactor::Sort(DoubleBuffer&, num_elements)."""
+    """radix_sort::Enactor::Sort(DoubleBuffer&, num_elements) for any key type
+    of the reference's KeyTraits (radix_sort/sort_utils.cuh:87-183), keys
+    only or with uint32 values.  The enactor owns its scratch (the spine)
+    across calls, as the reference Enactor owns its spine storage."""
+
+    def __init__(self):
+        h = ctypes.c_void_p()
+        check(lib().b40c_sort_enactor_create(ctypes.byref(h)))
+        self._h = h
 
     def Sort(self, storage, num_elements, begin_bit=0, end_bit=None):
         sel = ctypes.c_int32()
-        kt = storage.key_dtype
+        kt = KEY_TYPES.get(storage.key_dtype)
+        if kt is None:
+            raise TypeError("key dtype %s is not a radix-sortable type" % storage.key_dtype)
         if end_bit is None:
-            end_bit = 8 * kt.itemsize
-        if kt == np.uint64 and storage.d_values is None:
-            st = lib().b40c_radix_sort_u64(storage.d_keys[0].ptr, storage.d_keys[1].ptr, num_elements,
-                                           begin_bit, end_bit, ctypes.byref(sel))
-        elif kt == np.uint32 and storage.d_values is not None:
-            st = lib().b40c_radix_sort_pairs_u32(storage.d_keys[0].ptr, storage.d_keys[1].ptr,
-                                                 storage.d_values[0].ptr, storage.d_values[1].ptr,
-                                                 num_elements, begin_bit, end_bit, ctypes.byref(sel))
-        else:
-            raise TypeError("supported: uint64 keys, or uint32 keys with uint32 values")
+            end_bit = 8 * storage.key_dtype.itemsize
+        v = storage.d_values
+        s = storage.selector
+        st = lib().b40c_sort_enactor_sort(self._h, kt, storage.d_keys[s].ptr, storage.d_keys[s ^ 1].ptr,
+                                          v[s].ptr if v else None, v[s ^ 1].ptr if v else None,
+                                          int(num_elements), int(begin_bit), int(end_bit), ctypes.byref(sel))
         if st == 0:
-            storage.selector = sel.value
+            storage.selector ^= sel.value
         return st
+
+    def close(self):
+        if self._h:
+            lib().b40c_sort_enactor_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def sort(keys, values=None, begin_bit=0, end_bit=None):
+    """Host convenience for any supported key dtype: upload, stable sort on
+    the GPU, download (keys, or (keys, values))."""
+    keys = np.ascontiguousarray(keys)
+    db = DoubleBuffer(keys.dtype, keys.size, with_values=values is not None)
+    db.d_keys[0].upload(keys)
+    if values is not None:
+        db.d_values[0].upload(np.ascontiguousarray(values, np.uint32))
+    check(Enactor().Sort(db, keys.size, begin_bit, end_bit))
+    out_k = db.d_keys[db.selector].download(keys.dtype, keys.size)
+    if values is None:
+        return out_k
+    return out_k, db.d_values[db.selector].download(np.uint32, keys.size)
 
 
 def sort_u64(keys, begin_bit=0, end_bit=64):

@@ -2,9 +2,10 @@
  * C++ facade of the GPU radix sort with the reference's names:
  *   util::DoubleBuffer<K, V>       b40c/util/multiple_buffering.cuh:65-150
  *   radix_sort::Enactor::Sort      b40c/radix_sort/enactor.cuh:153-266
- * Keys: uint64 (keys only) or uint32 with uint32 values.  Both buffers of a
- * DoubleBuffer must be allocated by the caller (the reference allocates the
- * second lazily); after Sort, selector names the buffer holding the result.
+ * Keys: any KeyTraits type (uint32, int32, float, uint64, int64, double),
+ * alone or with 32-bit values.  Both buffers of a DoubleBuffer must be
+ * allocated by the caller (the reference allocates the second lazily); after
+ * Sort, selector names the buffer holding the result.
  * Digits are 8 bits wide (the reference uses 4); the sort is a stable LSD
  * sort either way, so results are identical.
  */
@@ -43,8 +44,26 @@ struct DoubleBuffer {
 
 namespace radix_sort {
 
+// Key type codes of the C ABI for the reference's KeyTraits key types
+// (radix_sort/sort_utils.cuh:163-183).
+template <typename K>
+struct KeyTypeCode;
+template <> struct KeyTypeCode<uint32_t> { static constexpr int32_t value = B40C_KEY_U32; };
+template <> struct KeyTypeCode<int32_t> { static constexpr int32_t value = B40C_KEY_I32; };
+template <> struct KeyTypeCode<float> { static constexpr int32_t value = B40C_KEY_F32; };
+template <> struct KeyTypeCode<uint64_t> { static constexpr int32_t value = B40C_KEY_U64; };
+template <> struct KeyTypeCode<int64_t> { static constexpr int32_t value = B40C_KEY_I64; };
+template <> struct KeyTypeCode<double> { static constexpr int32_t value = B40C_KEY_F64; };
+
+// Owns its scratch (the spine) across calls, like the reference Enactor
+// (radix_sort/enactor.cuh:96-140).  Not copyable.
 class Enactor {
 public:
+    Enactor() { (void)b40c_sort_enactor_create(&e_); }
+    ~Enactor() { b40c_sort_enactor_destroy(e_); }
+    Enactor(const Enactor&) = delete;
+    Enactor& operator=(const Enactor&) = delete;
+
     // Sorts bits [START_BIT, START_BIT + NUM_BITS) of the keys, stable.
     template <int START_BIT, int NUM_BITS, typename K, typename V, typename SizeT>
     b40c_status Sort(util::DoubleBuffer<K, V>& storage, SizeT num_elements, int max_grid_size = 0) {
@@ -62,24 +81,22 @@ public:
 private:
     template <typename K, typename V>
     b40c_status sort(util::DoubleBuffer<K, V>& s, int64_t n, int begin_bit, int end_bit) {
+        if (!e_) return B40C_ERR_HIP;
         int32_t sel = 0;
-        b40c_status st;
-        if constexpr (std::is_same<V, util::NullType>::value) {
-            static_assert(sizeof(K) == 8, "keys-only sorts take 64-bit keys");
-            st = b40c_radix_sort_u64(reinterpret_cast<uint64_t*>(s.d_keys[s.selector]),
-                                     reinterpret_cast<uint64_t*>(s.d_keys[s.selector ^ 1]), n, begin_bit, end_bit,
-                                     &sel);
-        } else {
-            static_assert(sizeof(K) == 4, "key-value sorts take 32-bit keys and values");
-            st = b40c_radix_sort_pairs_u32(reinterpret_cast<uint32_t*>(s.d_keys[s.selector]),
-                                           reinterpret_cast<uint32_t*>(s.d_keys[s.selector ^ 1]),
-                                           reinterpret_cast<uint32_t*>(s.d_values[s.selector]),
-                                           reinterpret_cast<uint32_t*>(s.d_values[s.selector ^ 1]), n, begin_bit,
-                                           end_bit, &sel);
+        uint32_t* v0 = nullptr;
+        uint32_t* v1 = nullptr;
+        if constexpr (!std::is_same<V, util::NullType>::value) {
+            static_assert(sizeof(V) == 4, "values are 32-bit");
+            v0 = reinterpret_cast<uint32_t*>(s.d_values[s.selector]);
+            v1 = reinterpret_cast<uint32_t*>(s.d_values[s.selector ^ 1]);
         }
+        const b40c_status st = b40c_sort_enactor_sort(e_, KeyTypeCode<K>::value, s.d_keys[s.selector],
+                                                      s.d_keys[s.selector ^ 1], v0, v1, n, begin_bit, end_bit, &sel);
         if (st == B40C_OK) s.selector ^= sel;
         return st;
     }
+
+    b40c_sort_enactor* e_ = nullptr;
 };
 
 }  // namespace radix_sort

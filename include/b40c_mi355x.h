@@ -180,6 +180,33 @@ b40c_status b40c_radix_sort_u64(uint64_t* d_keys0, uint64_t* d_keys1, int64_t n,
 b40c_status b40c_radix_sort_pairs_u32(uint32_t* d_keys0, uint32_t* d_keys1, uint32_t* d_vals0,
                                       uint32_t* d_vals1, int64_t n, int32_t begin_bit, int32_t end_bit,
                                       int32_t* selector);
+/* Key types of the generic sort: the reference's KeyTraits
+ * (b40c/radix_sort/sort_utils.cuh:87-183).  Signed keys are ordered by
+ * flipping the sign bit, IEEE floats by flipping the sign bit of positive
+ * values and every bit of negative ones (so -0.0 sorts before +0.0 and NaNs
+ * by their bits); begin_bit/end_bit index those ordered bits.  The keys in
+ * memory are never rewritten by the conversion. */
+#define B40C_KEY_U32 0
+#define B40C_KEY_I32 1
+#define B40C_KEY_F32 2
+#define B40C_KEY_U64 3
+#define B40C_KEY_I64 4
+#define B40C_KEY_F64 5
+/* radix_sort::Enactor::Sort over DoubleBuffer<K, V> for any key type K above,
+ * keys only (d_vals0 = d_vals1 = NULL) or with uint32 values
+ * (radix_sort/enactor.cuh:153-266).  Scratch is allocated for the call. */
+b40c_status b40c_radix_sort(int32_t key_type, void* d_keys0, void* d_keys1, uint32_t* d_vals0, uint32_t* d_vals1,
+                            int64_t n, int32_t begin_bit, int32_t end_bit, int32_t* selector);
+/* A sort enactor owns its scratch (the spine) across calls, like the
+ * reference Enactor's spine storage (radix_sort/enactor.cuh:96-140): a sort
+ * no larger than an earlier one on the same enactor allocates nothing.  One
+ * HIP stream per enactor; calls are synchronous (the selector is read back). */
+typedef struct b40c_sort_enactor b40c_sort_enactor;
+b40c_status b40c_sort_enactor_create(b40c_sort_enactor** out);
+b40c_status b40c_sort_enactor_sort(b40c_sort_enactor* e, int32_t key_type, void* d_keys0, void* d_keys1,
+                                   uint32_t* d_vals0, uint32_t* d_vals1, int64_t n, int32_t begin_bit,
+                                   int32_t end_bit, int32_t* selector);
+void b40c_sort_enactor_destroy(b40c_sort_enactor* e);
 /* Device allocation helpers so FFI callers can stage radix-sort buffers. */
 b40c_status b40c_device_alloc(void** d_ptr, int64_t bytes);
 b40c_status b40c_device_free(void* d_ptr);

@@ -11,7 +11,9 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <numeric>
+#include <type_traits>
 #include <vector>
 
 #include "b40c/primitives.hpp"
@@ -84,6 +86,80 @@ bool pairs_case(size_t n, uint64_t seed) {
     return ok;
 }
 
+// Keys of the reference's other KeyTraits types (sort_utils.cuh:163-183),
+// optionally with values, against std::stable_sort by the type's own `<`
+// (values are the input positions, so stability is checked too).  Floating
+// keys are finite and nonzero here, where `<` is a total order.
+template <typename K>
+K random_key(uint64_t& seed) {
+    const uint64_t r = splitmix(seed);
+    if constexpr (std::is_floating_point<K>::value)
+        return (K)((int64_t)(r >> 12) - (int64_t)(1ll << 51)) * (K)1e-9 + ((r & 1) ? (K)0.5 : (K)-0.5);
+    else
+        return (K)(r % 5000) - (K)(std::is_signed<K>::value ? 2500 : 0);  // duplicates, both signs
+}
+
+template <typename K>
+bool typed_case(const char* name, size_t n, uint64_t seed, bool with_values) {
+    std::vector<K> k(n);
+    std::vector<uint32_t> v(n);
+    for (size_t i = 0; i < n; ++i) {
+        k[i] = random_key<K>(seed);
+        v[i] = (uint32_t)i;
+    }
+    K* dk0 = device_copy(k);
+    K* dk1 = device_copy(std::vector<K>(n));
+    uint32_t* dv0 = device_copy(v);
+    uint32_t* dv1 = device_copy(std::vector<uint32_t>(n));
+    b40c::radix_sort::Enactor sorter;
+    b40c_status s;
+    std::vector<K> gk;
+    std::vector<uint32_t> gv;
+    if (with_values) {
+        b40c::util::DoubleBuffer<K, uint32_t> buf(dk0, dk1, dv0, dv1);
+        s = sorter.Sort(buf, n);
+        gk = host_copy(buf.d_keys[buf.selector], n);
+        gv = host_copy(buf.d_values[buf.selector], n);
+    } else {
+        b40c::util::DoubleBuffer<K> buf(dk0, dk1);
+        s = sorter.Sort(buf, n);
+        gk = host_copy(buf.d_keys[buf.selector], n);
+    }
+    for (void* p : {(void*)dk0, (void*)dk1, (void*)dv0, (void*)dv1}) b40c_device_free(p);
+    std::vector<size_t> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return k[a] < k[b]; });
+    bool ok = s == B40C_OK;
+    for (size_t i = 0; ok && i < n; ++i) ok = gk[i] == k[order[i]] && (!with_values || gv[i] == v[order[i]]);
+    printf("%s%s n=%zu: %s\n", name, with_values ? " pairs" : " keys", n, ok ? "CORRECT" : "INCORRECT");
+    return ok;
+}
+
+// IEEE special values, in the order the reference's float conversion gives
+// them (sort_utils.cuh:142-160: negative keys have every bit flipped,
+// positive keys the sign bit), worked out by hand from the bit patterns:
+// -NaN, -inf, -1, -0, +0, 1, inf, NaN.
+bool float_specials_case() {
+    const uint32_t bits[8] = {0x3F800000u, 0x80000000u, 0x7FC00000u, 0x00000000u,
+                              0xFF800000u, 0x7F800000u, 0xBF800000u, 0xFFC00000u};
+    const uint32_t want[8] = {0xFFC00000u, 0xFF800000u, 0xBF800000u, 0x80000000u,
+                              0x00000000u, 0x3F800000u, 0x7F800000u, 0x7FC00000u};
+    std::vector<float> k(8);
+    std::memcpy(k.data(), bits, sizeof(bits));
+    float* d0 = device_copy(k);
+    float* d1 = device_copy(std::vector<float>(8));
+    b40c::util::DoubleBuffer<float> buf(d0, d1);
+    b40c::radix_sort::Enactor sorter;
+    const b40c_status s = sorter.Sort(buf, 8);
+    std::vector<float> got = host_copy(buf.d_keys[buf.selector], 8);
+    b40c_device_free(d0);
+    b40c_device_free(d1);
+    bool ok = s == B40C_OK;
+    for (int i = 0; ok && i < 8; ++i) ok = std::memcmp(&got[i], &want[i], 4) == 0;
+    printf("float specials: %s\n", ok ? "CORRECT" : "INCORRECT");
+    return ok;
+}
+
 // Sorted keys with runs, trimmed with values; reference = first of each run.
 bool trim_case(size_t n, uint64_t seed) {
     std::vector<uint64_t> k(n);
@@ -151,6 +227,12 @@ int main(int argc, char** argv) {
     ok &= pairs_case(0, seed + 4);
     ok &= trim_case(n, seed + 5);
     ok &= trim_case(1, seed + 6);
+    ok &= typed_case<int32_t>("int32", n, seed + 9, true);
+    ok &= typed_case<int64_t>("int64", n, seed + 10, false);
+    ok &= typed_case<float>("float", n, seed + 11, false);
+    ok &= typed_case<double>("double", n, seed + 12, true);
+    ok &= typed_case<uint32_t>("uint32", n, seed + 13, false);
+    ok &= float_specials_case();
     ok &= scan_case(n, seed + 7);
     ok &= scan_case(0, seed + 8);
     printf("summary: %s\n", ok ? "all CORRECT" : "FAILURES");

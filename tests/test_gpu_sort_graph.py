@@ -51,6 +51,65 @@ def test_sort_pairs_u32_is_stable(rs, orc, n):
     np.testing.assert_array_equal(sv, ov)
 
 
+def random_keys(dtype, n, rng):
+    dt = np.dtype(dtype)
+    if dt.kind == "f":  # finite, both signs, many exact ties
+        return (rng.integers(-3000, 3000, n) * 0.25).astype(dt)
+    if dt.kind == "i":
+        info = np.iinfo(dt)
+        return rng.integers(info.min, info.max, n, dtype=dt, endpoint=True)
+    return rng.integers(0, 5000, n).astype(dt)
+
+
+@pytest.mark.parametrize("dtype", [np.int32, np.int64, np.float32, np.float64, np.uint32, np.uint64])
+@pytest.mark.parametrize("with_values", [False, True])
+def test_sort_key_types_against_numpy(rs, dtype, with_values):
+    """Every KeyTraits key type (radix_sort/sort_utils.cuh:163-183), keys
+    alone or with values, against numpy's stable argsort of the same keys:
+    the signed and float bit conversions must give the types' own order."""
+    for n in (1, 2049, 250001):
+        rng = np.random.default_rng(n + np.dtype(dtype).itemsize)
+        k = random_keys(dtype, n, rng)
+        order = np.argsort(k, kind="stable")
+        print("sort %s n=%d values=%s" % (np.dtype(dtype).name, n, with_values), flush=True)
+        if with_values:
+            v = np.arange(n, dtype=np.uint32)
+            sk, sv = rs.sort(k, v)
+            np.testing.assert_array_equal(sv, order.astype(np.uint32))
+        else:
+            sk = rs.sort(k)
+        np.testing.assert_array_equal(sk.view(np.dtype(dtype)), k[order])
+
+
+def test_sort_float_special_values(rs):
+    """-NaN, -inf, -1, -0, +0, 1, inf, NaN: the order the reference's float
+    conversion gives (every bit of a negative key flipped, the sign bit of a
+    positive one), worked out by hand from the bit patterns."""
+    bits = np.array([0x3F800000, 0x80000000, 0x7FC00000, 0x00000000,
+                     0xFF800000, 0x7F800000, 0xBF800000, 0xFFC00000], np.uint32)
+    want = np.array([0xFFC00000, 0xFF800000, 0xBF800000, 0x80000000,
+                     0x00000000, 0x3F800000, 0x7F800000, 0x7FC00000], np.uint32)
+    got = rs.sort(bits.view(np.float32))
+    np.testing.assert_array_equal(got.view(np.uint32), want)
+    got64 = rs.sort(np.array([1.0, -0.0, 0.0, -np.inf, -2.5], np.float64))
+    np.testing.assert_array_equal(got64.view(np.uint64),
+                                  np.array([-np.inf, -2.5, -0.0, 0.0, 1.0], np.float64).view(np.uint64))
+
+
+def test_sort_enactor_reuses_scratch(rs, orc):
+    """One enactor sorting a large then a smaller input (its scratch is kept
+    and reused), and the stateless entry point on the same keys."""
+    en = rs.Enactor()
+    rng = np.random.default_rng(11)
+    for n in (300000, 70000):
+        k = rng.integers(0, 2**64 - 1, n, dtype=np.uint64)
+        db = rs.DoubleBuffer(np.uint64, n)
+        db.d_keys[0].upload(k)
+        assert en.Sort(db, n) == 0
+        np.testing.assert_array_equal(db.d_keys[db.selector].download(np.uint64, n), orc.sort_u64(k))
+        np.testing.assert_array_equal(rs.sort_u64(k), orc.sort_u64(k))
+
+
 @pytest.mark.parametrize("scale,ef,seed", [(6, 4, 1), (10, 16, 0), (14, 16, 5), (16, 16, 0)])
 def test_rmat_csr_bit_exact(G, orc, scale, ef, seed):
     print("rmat scale=%d ef=%d seed=%d" % (scale, ef, seed), flush=True)
