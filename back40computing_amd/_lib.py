@@ -137,6 +137,7 @@ SIGNATURES = {
     "b40c_radix_sort_u64": (_i32, [_vp, _vp, _i64, _i32, _i32, _p_i32]),
     "b40c_radix_sort_pairs_u32": (_i32, [_vp, _vp, _vp, _vp, _i64, _i32, _i32, _p_i32]),
     "b40c_radix_sort": (_i32, [_i32, _vp, _vp, _vp, _vp, _i64, _i32, _i32, _p_i32]),
+    "b40c_radix_sort_u64_noalloc": (_i32, [_vp, _vp, _i64, _i32, _i32, _p_i32]),
     "b40c_sort_enactor_create": (_i32, [_pp]),
     "b40c_sort_enactor_sort": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, _i64, _i32, _i32, _p_i32]),
     "b40c_sort_enactor_destroy": (None, [_vp]),

@@ -89,7 +89,8 @@ template <int KIND>
 __global__ __launch_bounds__(BFS_THREADS) void k_fused(BfsArgs a, int ph, int32_t depth, int max_levels,
                                                        uint32_t sat_limit) {
     __shared__ LevelShared s;
-    fused_body<KIND>(a, ph, depth, max_levels, sat_limit, s);
+    __shared__ typename FusedHeavy<KIND>::shared hs;
+    fused_body<KIND>(a, ph, depth, max_levels, sat_limit, s, hs);
 }
 
 __global__ __launch_bounds__(BFS_THREADS) void k_gbarrier_bench(GridBarrier bar, int iterations) {

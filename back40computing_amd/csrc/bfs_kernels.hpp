@@ -952,12 +952,24 @@ __device__ void heavy_body(BfsArgs a, int ph, int32_t depth_next, int slot, type
 // workgroup and shared through LDS after each barrier; every workgroup reads
 // the same value, so all leave the loop together.  With `sat_limit` set, the
 // kernel quits when a frontier exceeds it (SATURATION_QUIT) and records
-// where it stopped.  The grid tier is not used here (callers pass grid_th =
-// INT_MAX): hub lists go to the workgroup tier.
+// where it stopped.
+//
+// Grid tier: a level's expansion defers lists of at least grid_th edges (as
+// the level kernels do); after the level's barrier every workgroup reads the
+// same deferral count, and when it is nonzero the whole grid splits those
+// lists into chunks (heavy_body) before one more barrier.  Levels without
+// hubs pay no extra barrier.  grid_th = INT_MAX turns the tier off.
 enum FusedKind { FK_TD = 0, FK_CE = 1, FK_TP = 2 };
 
 template <int KIND>
-__device__ void fused_body(BfsArgs a, int ph, int32_t depth, int max_levels, uint32_t sat_limit, LevelShared& s) {
+struct FusedHeavy {
+    static constexpr int kind = (KIND == FK_TD) ? HK_TD : HK_PAIRS;
+    typedef typename HeavySharedOf<kind>::type shared;
+};
+
+template <int KIND>
+__device__ void fused_body(BfsArgs a, int ph, int32_t depth, int max_levels, uint32_t sat_limit, LevelShared& s,
+                           typename FusedHeavy<KIND>::shared& hs) {
     unsigned long long t0 = 0, busy = 0;
     if (a.wg_stats && threadIdx.x == 0) t0 = wg_clock();
     shared_init(s);
@@ -979,8 +991,11 @@ __device__ void fused_body(BfsArgs a, int ph, int32_t depth, int max_levels, uin
         }
         unsigned long long tb = 0;
         if (a.wg_stats && threadIdx.x == 0) tb = wg_clock();
+        int heavy_ph = ph;          // phase whose deferred lists the grid tier takes
+        int32_t heavy_depth = depth;  // depth label of what it discovers (top-down)
         if (KIND == FK_TD) {
             td_phase(a, ph, depth + 1, nullptr, false, s.q, s.g, s.red, gridDim.x, a.grid_th);
+            heavy_depth = depth + 1;
             ph += 1;
         } else if (KIND == FK_CE) {
             ce_phase(a, ph, depth, s);
@@ -991,11 +1006,24 @@ __device__ void fused_body(BfsArgs a, int ph, int32_t depth, int max_levels, uin
             if (!grid_sync(bar, ++epoch)) break;
             if (a.wg_stats && threadIdx.x == 0) tb = wg_clock();
             tp_expand_phase(a, ph + 1, s);
+            heavy_ph = ph + 1;
             ph += 2;
         }
         depth += 1;
         if (a.wg_stats && threadIdx.x == 0) busy += wg_clock() - tb;
         if (!grid_sync(bar, ++epoch)) break;
+        if (a.grid_th != INT_MAX) {
+            if (threadIdx.x == 0)
+                s.ctl = __hip_atomic_load(&a.ctr->heavy_alloc[heavy_ph & 1], __ATOMIC_RELAXED,
+                                          __HIP_MEMORY_SCOPE_AGENT) != 0ull;
+            __syncthreads();
+            const bool deferred = s.ctl != 0;
+            __syncthreads();
+            if (deferred) {
+                heavy_body<FusedHeavy<KIND>::kind>(a, heavy_ph, heavy_depth, 0, hs);
+                if (!grid_sync(bar, ++epoch)) break;
+            }
+        }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         a.ctr->levels_run = (uint32_t)level;

@@ -54,7 +54,8 @@ SortScratch sort_scratch_view(uint32_t* words, size_t count);
 // Launches the passes (no allocation, no synchronization); the selector is
 // left in *sc.sel.  `launched` is false when there was nothing to sort.
 b40c_status radix_sort_async(int32_t key_type, void* k0, void* k1, uint32_t* v0, uint32_t* v1, int64_t n,
-                             int begin_bit, int end_bit, const SortScratch& sc, hipStream_t stream, bool& launched);
+                             int begin_bit, int end_bit, int num_cus, const SortScratch& sc, hipStream_t stream,
+                             bool& launched);
 b40c_status radix_sort_u64(uint64_t* k0, uint64_t* k1, int64_t n, int begin_bit, int end_bit, int32_t* selector,
                            hipStream_t stream);
 b40c_status radix_sort_pairs_u32(uint32_t* k0, uint32_t* k1, uint32_t* v0, uint32_t* v1, int64_t n, int begin_bit,

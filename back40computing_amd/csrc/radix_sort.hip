@@ -310,15 +310,14 @@ SortScratch sort_scratch_view(uint32_t* words, size_t count) {
 }
 
 b40c_status radix_sort_async(int32_t key_type, void* k0, void* k1, uint32_t* v0, uint32_t* v1, int64_t n,
-                             int begin_bit, int end_bit, const SortScratch& sc, hipStream_t stream, bool& launched) {
+                             int begin_bit, int end_bit, int num_cus, const SortScratch& sc, hipStream_t stream,
+                             bool& launched) {
     launched = false;
     if (n <= 1 || begin_bit == end_bit) return B40C_OK;
-    DeviceInfo info;
-    B40C_TRY(device_info(info));
-    const SortPlan plan = sort_plan(n, info.num_cus);
-    if (sort_scratch_words(n, info.num_cus) > sc.words)
+    const SortPlan plan = sort_plan(n, num_cus);
+    if (sort_scratch_words(n, num_cus) > sc.words)
         return fail(B40C_ERR_INVALID_ARG, "radix sort: scratch of %zu words is smaller than the %zu needed", sc.words,
-                    sort_scratch_words(n, info.num_cus));
+                    sort_scratch_words(n, num_cus));
     switch (key_type) {
         case B40C_KEY_U32: launch_typed<uint32_t, KC_UNSIGNED>(k0, k1, v0, v1, n, begin_bit, end_bit, plan, sc, stream); break;
         case B40C_KEY_I32: launch_typed<uint32_t, KC_SIGNED>(k0, k1, v0, v1, n, begin_bit, end_bit, plan, sc, stream); break;
@@ -336,9 +335,9 @@ b40c_status radix_sort_async(int32_t key_type, void* k0, void* k1, uint32_t* v0,
 // Runs a sort with its scratch and reads the selector back (one stream
 // synchronization).
 b40c_status radix_sort_sync(int32_t key_type, void* k0, void* k1, uint32_t* v0, uint32_t* v1, int64_t n, int begin_bit,
-                            int end_bit, const SortScratch& sc, int32_t* selector, hipStream_t stream) {
+                            int end_bit, int num_cus, const SortScratch& sc, int32_t* selector, hipStream_t stream) {
     bool launched = false;
-    B40C_TRY(radix_sort_async(key_type, k0, k1, v0, v1, n, begin_bit, end_bit, sc, stream, launched));
+    B40C_TRY(radix_sort_async(key_type, k0, k1, v0, v1, n, begin_bit, end_bit, num_cus, sc, stream, launched));
     int32_t h_sel = 0;
     if (launched) {
         B40C_HIP(hipMemcpyAsync(&h_sel, sc.sel, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
@@ -358,8 +357,8 @@ b40c_status radix_sort_once(int32_t key_type, void* k0, void* k1, uint32_t* v0, 
     DeviceBuffer mem;
     const size_t words = sort_scratch_words(n, info.num_cus);
     B40C_TRY(mem.alloc(words * sizeof(uint32_t)));
-    return radix_sort_sync(key_type, k0, k1, v0, v1, n, begin_bit, end_bit, sort_scratch_view(mem.as<uint32_t>(), words),
-                           selector, stream);
+    return radix_sort_sync(key_type, k0, k1, v0, v1, n, begin_bit, end_bit, info.num_cus,
+                           sort_scratch_view(mem.as<uint32_t>(), words), selector, stream);
 }
 
 b40c_status radix_sort_u64(uint64_t* k0, uint64_t* k1, int64_t n, int begin_bit, int end_bit, int32_t* selector,
@@ -374,6 +373,57 @@ b40c_status radix_sort_pairs_u32(uint32_t* k0, uint32_t* k1, uint32_t* v0, uint3
     return radix_sort_once(B40C_KEY_U32, k0, k1, v0, v1, n, begin_bit, end_bit, selector, stream);
 }
 
+}  // namespace b40c
+
+// ---------------------------------------------------------------------------
+// Allocation-free sort (no counterpart in the reference): its scratch is this
+// fixed device array, one copy per GPU, which the GPU keeps until the
+// process ends.  The grid is capped at RS_WGS_PER_CU workgroups per CU, so
+// kNoAllocWords covers a sort of any n <= 2^32 - 1 on a device with up to
+// kNoAllocMaxCus CUs (256 on MI355X): 1 MiB + 2 KiB.  In host code `scratch`
+// is only a stand-in for the device array; its device address comes from
+// hipGetSymbolAddress, once per device, and only that address is used.
+// ---------------------------------------------------------------------------
+namespace b40c {
+namespace {
+constexpr int kNoAllocMaxCus = 256;
+constexpr size_t kNoAllocWords = (size_t)RS_BINS * RS_WGS_PER_CU * kNoAllocMaxCus + 2 * RS_BINS + 64;
+}  // namespace
+}  // namespace b40c
+
+static __device__ __align__(256) uint32_t scratch[b40c::kNoAllocWords];
+
+namespace b40c {
+namespace {
+constexpr int kNoAllocMaxDevices = 64;
+struct NoAllocSlot {
+    std::once_flag once;
+    hipError_t err = hipSuccess;
+    uint32_t* ptr = nullptr;  // device address of `scratch` on this device
+    int num_cus = 0;
+};
+NoAllocSlot g_noalloc[kNoAllocMaxDevices];
+
+// Resolves the calling thread's current device's copy of `scratch` (first
+// call on a device only).
+b40c_status noalloc_slot(NoAllocSlot*& out) {
+    int dev = 0;
+    B40C_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoAllocMaxDevices)
+        return fail(B40C_ERR_UNSUPPORTED, "allocation-free sort: device %d beyond the %d supported", dev,
+                    kNoAllocMaxDevices);
+    NoAllocSlot& slot = g_noalloc[dev];
+    std::call_once(slot.once, [&] {
+        slot.err = hipGetSymbolAddress((void**)&slot.ptr, HIP_SYMBOL(scratch));
+        if (slot.err == hipSuccess) slot.err = hipDeviceGetAttribute(&slot.num_cus, hipDeviceAttributeMultiprocessorCount, dev);
+    });
+    if (slot.err != hipSuccess)
+        return fail(B40C_ERR_HIP, "allocation-free sort: resolving the scratch array failed: %s",
+                    hipGetErrorString(slot.err));
+    out = &slot;
+    return B40C_OK;
+}
+}  // namespace
 }  // namespace b40c
 
 // Sort enactor: owns its scratch across calls, like the reference Enactor's
@@ -409,6 +459,23 @@ b40c_status b40c_radix_sort(int32_t key_type, void* d_keys0, void* d_keys1, uint
     return radix_sort_once(key_type, d_keys0, d_keys1, d_vals0, d_vals1, n, begin_bit, end_bit, selector, nullptr);
 }
 
+b40c_status b40c_radix_sort_u64_noalloc(uint64_t* d_keys0, uint64_t* d_keys1, int64_t n, int32_t begin_bit,
+                                        int32_t end_bit, int32_t* selector) {
+    B40C_TRY(validate("b40c_radix_sort_u64_noalloc", B40C_KEY_U64, d_keys0, d_keys1, nullptr, nullptr, n, begin_bit,
+                      end_bit, selector));
+    *selector = 0;
+    if (n <= 1 || begin_bit == end_bit) return B40C_OK;
+    NoAllocSlot* slot = nullptr;
+    B40C_TRY(noalloc_slot(slot));
+    const size_t need = sort_scratch_words(n, slot->num_cus);
+    if (need > kNoAllocWords)
+        return fail(B40C_ERR_UNSUPPORTED,
+                    "b40c_radix_sort_u64_noalloc: n=%lld on %d CUs needs %zu scratch words; the fixed array holds %zu",
+                    (long long)n, slot->num_cus, need, kNoAllocWords);
+    return radix_sort_sync(B40C_KEY_U64, d_keys0, d_keys1, nullptr, nullptr, n, begin_bit, end_bit, slot->num_cus,
+                           sort_scratch_view(slot->ptr, kNoAllocWords), selector, nullptr);
+}
+
 b40c_status b40c_sort_enactor_create(b40c_sort_enactor** out) {
     if (!out) return fail(B40C_ERR_INVALID_ARG, "b40c_sort_enactor_create: null out");
     b40c_sort_enactor* e = new b40c_sort_enactor();
@@ -434,7 +501,7 @@ b40c_status b40c_sort_enactor_sort(b40c_sort_enactor* e, int32_t key_type, void*
     const size_t words = sort_scratch_words(n, info.num_cus);
     // Grow-only: later sorts of at most this size allocate nothing.
     B40C_TRY(e->scratch.reserve(words * sizeof(uint32_t)));
-    return radix_sort_sync(key_type, d_keys0, d_keys1, d_vals0, d_vals1, n, begin_bit, end_bit,
+    return radix_sort_sync(key_type, d_keys0, d_keys1, d_vals0, d_vals1, n, begin_bit, end_bit, info.num_cus,
                            sort_scratch_view(e->scratch.as<uint32_t>(), e->scratch.bytes / sizeof(uint32_t)), selector,
                            e->stream);
 }

@@ -136,17 +136,21 @@ def bench_lattice(bfs, b4, hip, info, width=200, sources=4, steps=8):
     srcs = pick_sources(off, 3, sources)
     out = {"workload": "lattice%d^3_perm_seed0" % width, "vertices": g.nodes, "csr_edges": g.edges,
            "sources": len(srcs)}
-    for s_id, name in ((b4.EXPAND_CONTRACT, "expand_contract"), (b4.HYBRID, "hybrid"),
-                       (b4.DIRECTION_OPTIMIZING, "direction_optimizing")):
+    for s_id, mode, name in ((b4.EXPAND_CONTRACT, b4.MODE_ITERATIVE, "expand_contract_iterative"),
+                             (b4.EXPAND_CONTRACT, b4.MODE_FUSED, "expand_contract_persistent"),
+                             (b4.CONTRACT_EXPAND, b4.MODE_FUSED, "contract_expand_persistent"),
+                             (b4.TWO_PHASE, b4.MODE_FUSED, "two_phase_persistent"),
+                             (b4.HYBRID, b4.MODE_DEFAULT, "hybrid"),
+                             (b4.DIRECTION_OPTIMIZING, b4.MODE_DEFAULT, "direction_optimizing")):
         t = tuning_for(s_id, g, info["arch"])
         e_trav, v_vis, levels = [], [], []
         for s in srcs:
-            depth, _, st = bfs.bfs(g, int(s), strategy=s_id, tuning=t)
+            depth, _, st = bfs.bfs(g, int(s), strategy=s_id, mode=mode, tuning=t)
             m = depth >= 0
             e_trav.append(int(deg[m].sum()))
             v_vis.append(int(m.sum()))
             levels.append(st["search_depth"])
-        r = measure_strategy(bfs, b4, hip, g, srcs, e_trav, v_vis, s_id, steps, 1, tuning=t)
+        r = measure_strategy(bfs, b4, hip, g, srcs, e_trav, v_vis, s_id, steps, 1, mode=mode, tuning=t)
         out[name] = {"gteps": round(r["gteps"], 3), "ms_per_search": round(r["ms_per_search_enact"], 3),
                      "levels_mean": round(float(np.mean(levels)), 1),
                      "us_per_level": round(1e3 * r["ms_per_search_enact"] / max(1.0, float(np.mean(levels))), 2),
@@ -453,6 +457,134 @@ def bench_primitives(hip, n_scan=1 << 28, n_trim=1 << 27, reps=5):
     return out
 
 
+def bench_sort(hip, n=1 << 27, reps=3):
+    """GPU radix sort at the CSR build's size (test/radix_sort/test_sort.cu:
+    155-163 reports 10^9 keys/s): n random 64-bit keys (8 digit places), and
+    n uint32 keys with uint32 values (4 places), through a sort enactor that
+    keeps its scratch.  Checked by size-independent properties: the output
+    is non-decreasing and has the input's key sum and xor (and, for pairs,
+    every value still travels with its key).  Algorithmic bytes: a key read
+    and written once per digit place (16 B per 64-bit key per place, 16 B per
+    key-value pair per place); this design moves 24 B per 64-bit key per
+    place (upsweep read, downsweep read and write)."""
+    from back40computing_amd import radix_sort as rs
+    rng = np.random.default_rng(5)
+    out = {}
+    en = rs.Enactor()
+    for name, kdt, with_vals, passes, alg_b in (("u64_keys", np.uint64, False, 8, 16.0),
+                                                 ("u32_pairs", np.uint32, True, 4, 16.0)):
+        info = np.iinfo(kdt)
+        keys = rng.integers(0, info.max, n, dtype=kdt, endpoint=True)
+        db = rs.DoubleBuffer(kdt, n, with_values=with_vals)
+        vals = None
+        if with_vals:
+            vals = (keys * np.uint32(2654435761)).astype(np.uint32)  # a function of the key, checkable after sort
+        best = None
+        for _ in range(reps):
+            db.selector = 0
+            db.d_keys[0].upload(keys)
+            if with_vals:
+                db.d_values[0].upload(vals)
+            hip.sync()
+            t0 = time.perf_counter()
+            st = en.Sort(db, n)
+            dt = time.perf_counter() - t0
+            assert st == 0
+            best = dt if best is None else min(best, dt)
+        got = db.d_keys[db.selector].download(kdt, n)
+        ok = bool(np.all(got[1:] >= got[:-1]))
+        ok = ok and int(got.sum(dtype=np.uint64)) == int(keys.sum(dtype=np.uint64))
+        ok = ok and int(np.bitwise_xor.reduce(got)) == int(np.bitwise_xor.reduce(keys))
+        if with_vals:
+            gv = db.d_values[db.selector].download(np.uint32, n)
+            ok = ok and bool(np.array_equal(gv, (got * np.uint32(2654435761)).astype(np.uint32)))
+        gbs = alg_b * passes * n / best / 1e9
+        out[name] = {"n": n, "passes": passes, "ms": round(best * 1e3, 3), "gkeys_per_s": round(n / best / 1e9, 3),
+                     "alg_gb_per_s": round(gbs, 1), "roofline_frac": round(gbs / HBM_PEAK_GBS, 4), "correct": ok}
+        log("sort %s: %s" % (name, out[name]))
+        del got, keys
+        for a in db.d_keys + (db.d_values or []):
+            a.free()
+    en.close()
+    out["note"] = ("host-timed synchronous sorts (selector read back), best of %d; alg bytes 16 B per key per "
+                   "digit place" % reps)
+    return out
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
+def cpu_baseline_leg(b4, bfs, strategy, tuning, graph22, off22, srcs22, scale22, budget_s=10.0, sources22=6):
+    """The reference's only CPU path, SimpleReferenceBfs (test/bfs/test_bfs.cu:
+    507-578), restated serially in oracle/oracle.cpp and timed on one host
+    core.  The designated measurement is the RMAT scale-16 graph: its 64
+    seeded sources are run round after round until `budget_s` of wall time,
+    with steady_clock wall time and getrusage user time per run (the
+    reference's CpuTimer, test/b40c_test_util.h:503-522).  The same oracle on
+    `sources22` sources of the RMAT-22 headline graph is reported beside it.
+    The GPU labels of every timed source are compared with the oracle's."""
+    import oracle
+    from back40computing_amd.sources import pick_sources
+    off16, cols16 = oracle.build_rmat_csr(16, 16, 0.57, 0.19, 0.19, 0)
+    g16 = bfs.Graph.rmat(16, 16, 0.57, 0.19, 0.19, 0)
+    o16, c16 = g16.to_host()
+    csr_same = bool(np.array_equal(o16, off16) and np.array_equal(c16, cols16))
+    srcs16 = [int(s) for s in pick_sources(off16, 1, 64)]
+    deg16 = np.diff(off16).astype(np.int64)
+    same16 = 0
+    tot_e = tot_wall = tot_user = 0.0
+    runs = 0
+    t_start = time.perf_counter()
+    first = True
+    while first or time.perf_counter() - t_start < budget_s:
+        for s in srcs16:
+            d, sd, wall_ms, user_ms = oracle.bfs_timed(off16, cols16, s)
+            tot_e += float(deg16[d >= 0].sum())
+            tot_wall += wall_ms
+            tot_user += user_ms
+            runs += 1
+            if first:
+                d_gpu, _, st = bfs.bfs(g16, s, strategy=strategy)
+                same16 += int(np.array_equal(d_gpu, d) and st["search_depth"] == sd)
+        first = False
+    g16.close()
+    out = {"value": round(tot_e / (tot_wall * 1e-3) / 1e9, 5), "unit": "GTEPS", "cores": 1, "kind": "port",
+           "sample": "serial FIFO BFS (SimpleReferenceBfs restated) on RMAT-16 (a=0.57 b=0.19 c=0.19, ef 16, "
+                     "sym, dedup, V=%d E=%d), its 64 seeded sources repeated to %d runs (%.1f s), steady_clock "
+                     "wall time, 1 thread" % (off16.size - 1, cols16.size, runs, tot_wall * 1e-3),
+           "user_time_gteps": round(tot_e / (max(tot_user, 1e-9) * 1e-3) / 1e9, 5),
+           "cpu_model": cpu_model(), "nproc": os.cpu_count(),
+           "gpu_csr_identical": csr_same, "gpu_labels_identical": "%d/%d" % (same16, len(srcs16))}
+    # Secondary: the headline graph.
+    if sources22 > 0:
+        E = graph22.edges
+        cols22 = np.empty(E, np.int32)
+        b4._lib.check(b4.lib().b40c_graph_copy_to_host(graph22.handle, None, b4._lib.ptr(cols22)))
+        deg22 = np.diff(off22).astype(np.int64)
+        te = tw = tu = 0.0
+        same22 = 0
+        for s in srcs22[:sources22]:
+            d, sd, wall_ms, user_ms = oracle.bfs_timed(off22, cols22, int(s))
+            te += float(deg22[d >= 0].sum())
+            tw += wall_ms
+            tu += user_ms
+            d_gpu, _, st = bfs.bfs(graph22, int(s), strategy=strategy, tuning=tuning)
+            same22 += int(np.array_equal(d_gpu, d) and st["search_depth"] == sd)
+        out["rmat%d" % scale22] = {"gteps_wall": round(te / (tw * 1e-3) / 1e9, 5),
+                                   "gteps_user": round(te / (max(tu, 1e-9) * 1e-3) / 1e9, 5),
+                                   "sources": min(sources22, len(srcs22)),
+                                   "gpu_labels_identical": "%d/%d" % (same22, min(sources22, len(srcs22)))}
+    return out
+
+
 def pick_sources_cached(off, seed, count):
     from back40computing_amd.sources import pick_sources
     return pick_sources(off, seed, count)
@@ -519,6 +651,22 @@ def measured_traffic(workload, strategy):
     return out
 
 
+def roofline_block(achieved_gbs, ms_per_search, workload, strategy):
+    """achieved/frac: algorithmic bytes (B_alg = 8 E_trav + 16 V_vis per
+    search) per second, so for a direction-optimizing search they also count
+    the traffic its bottom-up steps avoid.  traffic_gbs/traffic_frac: the
+    HBM bytes a search really moves (rocprofv3 counters, committed summary)
+    over the same measured search time."""
+    out = {"bound": "hbm", "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved_gbs / HBM_PEAK_GBS, 4), "achieved_kind": "algorithmic bytes (B_alg) per second"}
+    out.update(measured_traffic(workload, strategy))
+    if out.get("traffic"):
+        tg = out["traffic"] / (ms_per_search * 1e-3) / 1e9
+        out["traffic_gbs"] = round(tg, 1)
+        out["traffic_frac"] = round(tg / HBM_PEAK_GBS, 4)
+    return out
+
+
 def run_child(which, timeout_s=420):
     """Runs `bench.py --only <which>` in a child process (its progress goes to
     our stderr) and returns its result dict, or an error record."""
@@ -545,7 +693,9 @@ def main():
     ap.add_argument("--sources", type=int, default=64)
     ap.add_argument("--strategy", type=int, default=5, help="headline strategy (5 = direction-optimizing)")
     ap.add_argument("--compare-steps", type=int, default=16, help="steps for the other strategies (0: skip)")
-    ap.add_argument("--cpu-sources", type=int, default=6, help="oracle BFS sources timed for cpu_baseline (0: skip)")
+    ap.add_argument("--cpu-sources", type=int, default=6,
+                    help="RMAT-22 oracle BFS sources timed beside the RMAT-16 cpu_baseline (0: skip the CPU baseline)")
+    ap.add_argument("--no-sort", action="store_true", help="skip the radix-sort workload")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched 1,024-query workload")
     ap.add_argument("--no-served", action="store_true", help="skip the served Poisson workload")
     ap.add_argument("--no-lattice", action="store_true", help="skip the 200^3 lattice (high-diameter) workload")
@@ -612,18 +762,24 @@ def main():
              5: "direction_optimizing"}
     per_strategy = {names[args.strategy]: round(head["gteps"], 3)}
     if args.compare_steps > 0:
-        for s in (0, 1, 2, 3, 4, 5):
-            if s == args.strategy:
+        # Each strategy in its default launch form (the reference's: fused for
+        # expand-contract and contract-expand, test_bfs.cu:690-718), plus the
+        # other form where both exist.
+        for s, mode, name in ((0, 0, "expand_contract"), (0, 2, "expand_contract_iterative"),
+                              (1, 0, "contract_expand"), (1, 2, "contract_expand_iterative"),
+                              (2, 0, "two_phase"), (2, 1, "two_phase_persistent"), (3, 0, "hybrid"),
+                              (5, 0, "direction_optimizing")):
+            if s == args.strategy and mode == 0:
                 continue
             try:
-                r = measure_strategy(bfs, b4, hip, graph, srcs, e_trav, v_vis, s, args.compare_steps, 2,
+                r = measure_strategy(bfs, b4, hip, graph, srcs, e_trav, v_vis, s, args.compare_steps, 2, mode=mode,
                                      tuning=tuning_for(s, graph, info["arch"]))
             except b4.B40CError as ex:
-                per_strategy[names[s]] = None
-                log("strategy %s: unavailable (%s)" % (names[s], ex))
+                per_strategy[name] = None
+                log("strategy %s: unavailable (%s)" % (name, ex))
                 continue
-            per_strategy[names[s]] = round(r["gteps"], 3)
-            log("strategy %s: %.3f GTEPS (%.3f ms/search)" % (names[s], r["gteps"], r["ms_per_search_enact"]))
+            per_strategy[name] = round(r["gteps"], 3)
+            log("strategy %s: %.3f GTEPS (%.3f ms/search)" % (name, r["gteps"], r["ms_per_search_enact"]))
 
     lattice = None
     if not args.no_lattice:
@@ -662,27 +818,19 @@ def main():
     batched = None if args.no_batched else run_child("batched")
     served = None if args.no_served else run_child("served")
 
-    # ---- CPU baseline: the serial oracle BFS on a bounded sample ----------
+    sort = None
+    if not args.no_sort:
+        try:
+            sort = bench_sort(hip)
+        except b4.B40CError as ex:
+            sort = {"error": str(ex)}
+            log("sort workload failed:", ex)
+
+    # ---- CPU baseline: the serial oracle BFS, RMAT-16 designated ----------
     cpu = None
     if args.cpu_sources > 0:
-        import oracle
-        cols = np.empty(E, np.int32)
-        b4._lib.check(b4.lib().b40c_graph_copy_to_host(graph.handle, None, b4._lib.ptr(cols)))
-        tot_e, tot_ms, same = 0, 0.0, 0
-        for s in srcs[:args.cpu_sources]:
-            d, sd, wall_ms, user_ms = oracle.bfs_timed(off, cols, int(s))
-            tot_e += int(deg[d >= 0].sum())
-            tot_ms += wall_ms
-            # The timed oracle run doubles as the checker of the headline
-            # strategy's labels for the same source.
-            d_gpu, _, st = bfs.bfs(graph, int(s), strategy=args.strategy, tuning=head_tuning)
-            same += int(np.array_equal(d_gpu, d) and st["search_depth"] == sd)
-        cpu = {"value": round(tot_e / (tot_ms * 1e-3) / 1e9, 5), "unit": "GTEPS", "cores": 1, "kind": "port",
-               "sample": "serial FIFO BFS (restated SimpleReferenceBfs) from the first %d of the %d sources on "
-                         "the same RMAT-%d CSR, steady_clock wall time, 1 thread" % (args.cpu_sources, len(srcs),
-                                                                                      args.scale),
-               "cpu": platform.processor() or platform.machine(), "nproc": os.cpu_count(),
-               "gpu_labels_identical": "%d/%d" % (same, min(args.cpu_sources, len(srcs)))}
+        cpu = cpu_baseline_leg(b4, bfs, args.strategy, head_tuning, graph, off, srcs, args.scale,
+                               sources22=args.cpu_sources)
         log("cpu baseline", cpu)
 
     achieved = head["achieved_gbs"]
@@ -704,15 +852,14 @@ def main():
                    "vertices": n, "csr_edges": E, "sources": len(srcs), "strategy": names[args.strategy],
                    "tuning": head_tuning or "library defaults",
                    "step": "Reset + EnactSearch, depth + predecessor output"},
-        "roofline": dict({"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                          "frac": round(achieved / HBM_PEAK_GBS, 4)},
-                         **measured_traffic(out_workload, names[args.strategy])),
+        "roofline": roofline_block(achieved, head["ms_per_search_enact"], out_workload, names[args.strategy]),
         "cpu_baseline": cpu,
         "lattice": lattice,
         "batched": batched,
         "served": served,
         "ingest": ingest,
         "primitives": prims,
+        "sort": sort,
         "microbench": micro,
         "detail": {"gteps_enact_only": round(head["gteps_enact_only"], 3),
                    "ms_per_search_enact": round(head["ms_per_search_enact"], 4),

@@ -180,6 +180,16 @@ b40c_status b40c_radix_sort_u64(uint64_t* d_keys0, uint64_t* d_keys1, int64_t n,
 b40c_status b40c_radix_sort_pairs_u32(uint32_t* d_keys0, uint32_t* d_keys1, uint32_t* d_vals0,
                                       uint32_t* d_vals1, int64_t n, int32_t begin_bit, int32_t end_bit,
                                       int32_t* selector);
+/* b40c_radix_sort_u64 without any allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code that must
+ * not call hipMalloc, tight serving loops, harnesses that forbid it).  The
+ * scratch is a fixed 1 MiB device array per GPU, held until the process
+ * ends, which covers any n <= 2^32 - 1 on a device of up to 256 CUs (larger
+ * requests fail with B40C_ERR_UNSUPPORTED).  All calls on one GPU share that
+ * array, so calls that overlap in time on one GPU (other host threads) race
+ * and must be serialized by the caller.  Synchronous, on the default stream. */
+b40c_status b40c_radix_sort_u64_noalloc(uint64_t* d_keys0, uint64_t* d_keys1, int64_t n, int32_t begin_bit,
+                                        int32_t end_bit, int32_t* selector);
 /* Key types of the generic sort: the reference's KeyTraits
  * (b40c/radix_sort/sort_utils.cuh:87-183).  Signed keys are ordered by
  * flipping the sign bit, IEEE floats by flipping the sign bit of positive

@@ -172,14 +172,26 @@ struct Replay {
                 heavy_body<HK_PAIRS>(a, ph, depth_next, 0, s);
             });
     }
+    template <int KIND>
+    struct FusedLds {
+        LevelShared s;
+        typename FusedHeavy<KIND>::shared hs;
+    };
     void fused(int kind, int ph, int32_t depth_arg, uint32_t sat) {
         BfsArgs fa = a;
-        fa.grid_th = INT_MAX;
-        run<LevelShared>(grid, [&](LevelShared& s) {
-            if (kind == FK_TD) fused_body<FK_TD>(fa, ph, depth_arg, 1 << 30, sat, s);
-            if (kind == FK_CE) fused_body<FK_CE>(fa, ph, depth_arg, 1 << 30, sat, s);
-            if (kind == FK_TP) fused_body<FK_TP>(fa, ph, depth_arg, 1 << 30, sat, s);
-        });
+        if (!heavy_needed) fa.grid_th = INT_MAX;
+        if (kind == FK_TD)
+            run<FusedLds<FK_TD>>(grid, [&](FusedLds<FK_TD>& l) {
+                fused_body<FK_TD>(fa, ph, depth_arg, 1 << 30, sat, l.s, l.hs);
+            });
+        if (kind == FK_CE)
+            run<FusedLds<FK_CE>>(grid, [&](FusedLds<FK_CE>& l) {
+                fused_body<FK_CE>(fa, ph, depth_arg, 1 << 30, sat, l.s, l.hs);
+            });
+        if (kind == FK_TP)
+            run<FusedLds<FK_TP>>(grid, [&](FusedLds<FK_TP>& l) {
+                fused_body<FK_TP>(fa, ph, depth_arg, 1 << 30, sat, l.s, l.hs);
+            });
     }
     bool stop(int ph) const { return ctr->overflow || ctr->corrupt || ctr->bar_abort || ctr->qlen[ph & 3] == 0; }
 };
