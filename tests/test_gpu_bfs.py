@@ -246,24 +246,33 @@ def test_from_host_problem_signature(bfsmod, orc):
 @pytest.fixture(scope="module")
 def rmat22(bfsmod, orc):
     """The headline graph at full size (RMAT scale 22, edge factor 16,
-    a=0.57 b=0.19 c=0.19, seed 0) and the first 8 of its 64 seeded sources."""
+    a=0.57 b=0.19 c=0.19, seed 0) and all 64 of its seeded sources, with the
+    oracle's depths for each."""
     g = bfsmod.Graph.rmat(22, 16, 0.57, 0.19, 0.19, 0)
     off, cols = g.to_host()
-    srcs = [int(s) for s in orc.pick_sources(off, 1, 64)[:8]]
-    refs = orc.bfs_many(off, cols, srcs)
+    srcs = [int(s) for s in orc.pick_sources(off, 1, 64)]
+    refs = orc.bfs_many(off, cols, srcs, threads=16)
     return g, off, cols, srcs, refs
 
 
-@pytest.mark.parametrize("strategy", [b4.DIRECTION_OPTIMIZING, b4.EXPAND_CONTRACT, b4.HYBRID])
-def test_rmat22_headline_sources(bfsmod, orc, rmat22, strategy):
+@pytest.mark.parametrize("strategy,mode,nsrc", [(b4.DIRECTION_OPTIMIZING, b4.MODE_DEFAULT, 64),
+                                                (b4.EXPAND_CONTRACT, b4.MODE_ITERATIVE, 64),
+                                                (b4.HYBRID, b4.MODE_DEFAULT, 64),
+                                                (b4.EXPAND_CONTRACT, b4.MODE_FUSED, 8),
+                                                (b4.CONTRACT_EXPAND, b4.MODE_FUSED, 8),
+                                                (b4.TWO_PHASE, b4.MODE_FUSED, 8),
+                                                (b4.TWO_PHASE, b4.MODE_ITERATIVE, 8)],
+                         ids=["5-0", "0-2", "3-0", "0-1", "1-1", "2-1", "2-2"])
+def test_rmat22_headline_sources(bfsmod, orc, rmat22, strategy, mode, nsrc):
     """Full-size parity on the headline workload, with the tuned knobs the
-    bench uses: depth bit-exact, every predecessor valid, search depth."""
+    bench uses: depth bit-exact, every predecessor valid, search depth.  The
+    headline strategy and the reference's default hybrid run all 64 sources."""
     from back40computing_amd import autotune
     g, off, cols, srcs, refs = rmat22
     info = bfsmod.device_info()
     tuning = autotune.lookup(info["arch"], strategy, g.nodes, g.edges)
-    for k, s in enumerate(srcs):
-        depth, pred, st = run(bfsmod, g, s, strategy, b4.MODE_DEFAULT, tuning=tuning)
+    for k, s in enumerate(srcs[:nsrc]):
+        depth, pred, st = run(bfsmod, g, s, strategy, mode, tuning=tuning)
         ref = refs[k]
         sd = int(ref.max()) + 1
         np.testing.assert_array_equal(depth, ref)
@@ -280,14 +289,26 @@ def lattice200(bfsmod):
     return g, perm, off, cols
 
 
-@pytest.mark.parametrize("strategy", [b4.EXPAND_CONTRACT, b4.HYBRID, b4.DIRECTION_OPTIMIZING])
-def test_lattice200_closed_form(bfsmod, orc, lattice200, strategy):
+@pytest.mark.parametrize("strategy,mode,nsrc", [(b4.EXPAND_CONTRACT, b4.MODE_FUSED, 16),
+                                                (b4.HYBRID, b4.MODE_DEFAULT, 16),
+                                                (b4.DIRECTION_OPTIMIZING, b4.MODE_DEFAULT, 16),
+                                                (b4.EXPAND_CONTRACT, b4.MODE_ITERATIVE, 4),
+                                                (b4.CONTRACT_EXPAND, b4.MODE_FUSED, 4),
+                                                (b4.TWO_PHASE, b4.MODE_FUSED, 4),
+                                                (b4.TWO_PHASE, b4.MODE_ITERATIVE, 4)],
+                         ids=["0-1", "3-0", "5-0", "0-2", "1-1", "2-1", "2-2"])
+def test_lattice200_closed_form(bfsmod, orc, lattice200, strategy, mode, nsrc):
     """Full-size lattice: depth equals the Manhattan distance on the
-    unpermuted coordinates (about 600 levels from a corner)."""
+    unpermuted coordinates (up to 598 levels from a corner).  Sources: two
+    corners and a middle vertex, then the lattice workload's seeded sources
+    (16 in all)."""
     g, perm, off, cols = lattice200
     assert g.nodes == 8000000 and g.edges == 47760000
-    for s in (int(perm[0]), int(perm[200 ** 3 // 2 + 100])):  # a corner and a middle vertex
-        depth, pred, st = run(bfsmod, g, s, strategy, b4.MODE_DEFAULT)
+    w = 200
+    fixed = [int(perm[0]), int(perm[w ** 3 - 1]), int(perm[w ** 3 // 2 + 100])]
+    seeded = [int(x) for x in orc.pick_sources(off, 3, 13)]
+    for s in (fixed + seeded)[:nsrc]:
+        depth, pred, st = run(bfsmod, g, s, strategy, mode)
         ref = orc.lattice_depth(200, perm, s)
         np.testing.assert_array_equal(depth, ref)
         assert orc.check_pred(off, cols, ref, pred) == -1

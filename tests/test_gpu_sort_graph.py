@@ -110,7 +110,7 @@ def test_sort_enactor_reuses_scratch(rs, orc):
         np.testing.assert_array_equal(rs.sort_u64(k), orc.sort_u64(k))
 
 
-@pytest.mark.parametrize("scale,ef,seed", [(6, 4, 1), (10, 16, 0), (14, 16, 5), (16, 16, 0)])
+@pytest.mark.parametrize("scale,ef,seed", [(6, 4, 1), (10, 16, 0), (14, 16, 5), (16, 16, 0), (22, 16, 0)])
 def test_rmat_csr_bit_exact(G, orc, scale, ef, seed):
     print("rmat scale=%d ef=%d seed=%d" % (scale, ef, seed), flush=True)
     g = G.rmat(scale, ef, 0.57, 0.19, 0.19, seed)

@@ -191,3 +191,22 @@ def test_work_histogram_invariants(orc):
     assert ex[1:].sum() == ev
     assert np.all(ue[1:] <= ex[1:])
     assert np.all(dc[1:] <= ue[1:])
+
+
+def test_bench_source_picker_matches_oracle(orc):
+    """The bench and the batched/served workloads pick sources with
+    back40computing_amd.sources.pick_sources; the GPU tests with the
+    oracle's orc_pick_sources.  Both must return the same sources (same
+    seeds, same order), including graphs with many isolated vertices and
+    requests for more sources than there are candidates."""
+    import numpy as np
+    from back40computing_amd.sources import pick_sources
+    off16, _ = orc.build_rmat_csr(16, 16, 0.57, 0.19, 0.19, 0)
+    off10, _ = orc.build_rmat_csr(10, 4, 0.57, 0.19, 0.19, 9)
+    sparse = np.array([0, 0, 2, 2, 2, 3, 3, 3, 4], np.int32)  # 3 vertices with edges out of 8
+    for off, seed, count in ((off16, 1, 64), (off16, 11, 300), (off10, 3, 50), (sparse, 5, 3), (sparse, 5, 10)):
+        a = pick_sources(off, seed, count)
+        b = orc.pick_sources(off, seed, count)
+        np.testing.assert_array_equal(a, b)
+        assert np.all(np.diff(off)[a] > 0) and len(set(a.tolist())) == a.size
+    assert pick_sources(sparse, 5, 10).size == 3
