@@ -6,8 +6,14 @@ b40c/radix_sort/autotuned_policy.cuh:58-659) with measurement: for a given
 graph, strategy and sample of sources it runs a coordinate descent over the
 runtime knobs of ``b40c_tuning`` (grid size, gather-tier thresholds,
 direction-switch parameters, levels per host check), timing each candidate
-with HIP events on the problem's stream and rejecting any candidate whose
-depth labels differ from the default configuration's.
+with HIP events on the problem's stream.
+
+Correctness of every accepted candidate: the default configuration's labels
+for the sample sources are first proven to be BFS labels by
+``verify_bfs_labels`` (local invariants that determine BFS depths uniquely,
+checked over every edge of the graph), and a candidate is rejected unless
+its depth labels equal those exactly and its predecessors pass the same
+predecessor checks.
 
 Results go to a JSON tuning table keyed by (gfx arch, strategy, graph class);
 ``lookup()`` returns the entry for a graph so callers can apply it with
@@ -52,6 +58,61 @@ KNOBS = {
     _lib.DIRECTION_OPTIMIZING: ["bottom_up_grid_size", "do_alpha", "do_beta", "grid_gather_threshold",
                                 "grid_size", "levels_per_sync", "wave_gather_threshold"],
 }
+
+
+def verify_bfs_labels(offsets, cols, src, depth, pred, edge_keys=None):
+    """True when (depth, pred) is a breadth-first labelling of the CSR graph
+    from `src`, checked by invariants that determine BFS depths uniquely:
+    depth[src] = 0 and pred[src] = -2; every other reached vertex v has
+    pred[v] = u with an edge u -> v and depth[u] = depth[v] - 1; no edge
+    leaves a reached vertex u for an unreached one or for one deeper than
+    depth[u] + 1; unreached vertices have depth and pred -1.  (By induction
+    on depth, the first two make depth[v] >= the true distance; the third
+    makes it <= the distance and the reached set exactly the vertices
+    reachable from src.)  `edge_keys` (row * n + col of every edge, sorted)
+    may be passed to reuse it across sources."""
+    off = np.asarray(offsets, np.int64)
+    n = off.size - 1
+    depth = np.asarray(depth)
+    pred = np.asarray(pred)
+    if depth.shape != (n,) or pred.shape != (n,) or not (0 <= src < n):
+        return False
+    if depth[src] != 0 or pred[src] != -2:
+        return False
+    reached = depth >= 0
+    if np.any(depth[~reached] != -1) or np.any(pred[~reached] != -1):
+        return False
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(off))
+    c = np.asarray(cols, np.int64)
+    du = depth[rows]
+    dv = depth[c]
+    from_reached = du >= 0
+    if np.any(dv[from_reached] < 0) or np.any(dv[from_reached] > du[from_reached] + 1):
+        return False
+    others = np.flatnonzero(reached)
+    others = others[others != src]
+    p = pred[others].astype(np.int64)
+    if np.any((p < 0) | (p >= n)):
+        return False
+    if np.any(depth[p] != depth[others] - 1):
+        return False
+    if others.size == 0:
+        return True
+    if edge_keys is None:
+        edge_keys = np.sort(rows * n + c)
+    if edge_keys.size == 0:
+        return False
+    want = p * n + others
+    pos = np.minimum(np.searchsorted(edge_keys, want), edge_keys.size - 1)
+    return bool(np.all(edge_keys[pos] == want))
+
+
+def sorted_edge_keys(offsets, cols):
+    """row * n + col of every edge, sorted (for verify_bfs_labels)."""
+    off = np.asarray(offsets, np.int64)
+    n = off.size - 1
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(off))
+    return np.sort(rows * n + np.asarray(cols, np.int64))
 
 
 def graph_class(nodes, edges):
@@ -99,9 +160,10 @@ class _Timer:
         return ms.value
 
 
-def measure(graph, strategy, sources, tuning, repeats=1, reference=None):
+def measure(graph, strategy, sources, tuning, repeats=1, reference=None, pred_check=None):
     """Mean milliseconds per search (Reset + EnactSearch) for `tuning`; returns
-    (ms, depths).  Raises if the labels differ from `reference`."""
+    (ms, depths, preds).  Raises if the depth labels differ from `reference`
+    or a predecessor array fails `pred_check(k, depth, pred)`."""
     from . import bfs
     prob = bfs.CsrProblem()
     check(prob.FromGraph(graph))
@@ -110,17 +172,22 @@ def measure(graph, strategy, sources, tuning, repeats=1, reference=None):
     ft = en.GetFrontierType()
     stream = prob.stream()
     timer = _Timer()
-    depths = []
+    depths, preds = [], []
     # correctness pass (also warms up)
     for s in sources:
         check(prob.Reset(ft, 1.3))
         check(lib().b40c_enact_search(en.handle, prob.handle, int(s), 0, _lib.MODE_DEFAULT))
-        d, _ = prob.results()
+        d, p = prob.results()
         depths.append(d)
+        preds.append(p)
     if reference is not None:
         for a, b in zip(depths, reference):
             if not np.array_equal(a, b):
                 raise AssertionError("tuning %s changed the depth labels" % (tuning,))
+    if pred_check is not None:
+        for k, (d, p) in enumerate(zip(depths, preds)):
+            if not pred_check(k, d, p):
+                raise AssertionError("tuning %s produced an invalid predecessor" % (tuning,))
     total = 0.0
     for _ in range(repeats):
         timer.start(stream)
@@ -130,30 +197,53 @@ def measure(graph, strategy, sources, tuning, repeats=1, reference=None):
         total += timer.stop_ms(stream)
     prob.close()
     en.close()
-    return total / (repeats * len(sources)), depths
+    return total / (repeats * len(sources)), depths, preds
 
 
-def tune(graph, strategy, sources, rounds=2, repeats=2, log=None):
-    """Coordinate descent from the library defaults.  Returns (tuning, ms, history)."""
+class LabelsInvalid(AssertionError):
+    pass
+
+
+def tune(graph, strategy, sources, rounds=2, repeats=2, log=None, knobs=None, space=None):
+    """Coordinate descent from the library defaults.  Returns (tuning, ms,
+    history).  The defaults' labels are verified as BFS labels first; every
+    candidate must reproduce them."""
     from . import bfs
+    space = space or SPACE
     en = bfs.ENACTORS[strategy]()
-    best = {k: v for k, v in en.tuning.items() if k in SPACE}
+    best = {k: v for k, v in en.tuning.items() if k in space}
     en.close()
-    best_ms, ref = measure(graph, strategy, sources, best, repeats)
+    off, cols = graph.to_host()
+    keys = sorted_edge_keys(off, cols)
+    best_ms, ref, ref_pred = measure(graph, strategy, sources, best, repeats)
+    for s, d, p in zip(sources, ref, ref_pred):
+        if not verify_bfs_labels(off, cols, int(s), d, p, keys):
+            raise LabelsInvalid("default tuning gave labels that are not a BFS from %d" % int(s))
+
+    def pred_ok(k, d, p):
+        return verify_bfs_labels(off, cols, int(sources[k]), d, p, keys)
+
     history = [(dict(best), best_ms)]
     if log:
         log("default %s: %.4f ms/search" % (best, best_ms))
     for _ in range(rounds):
         improved = False
-        for knob in KNOBS[strategy]:
-            for val in SPACE[knob]:
+        for knob in (knobs or KNOBS[strategy]):
+            for val in space[knob]:
                 if val == best.get(knob):
                     continue
                 cand = dict(best)
                 cand[knob] = val
                 try:
-                    ms, _ = measure(graph, strategy, sources, cand, repeats, reference=ref)
+                    ms, _, _ = measure(graph, strategy, sources, cand, repeats, reference=ref, pred_check=pred_ok)
                 except _lib.B40CError:
+                    continue
+                except AssertionError as ex:
+                    # Wrong labels from a candidate are a library bug: never
+                    # accept it, and say so.
+                    history.append((dict(cand), None))
+                    if log:
+                        log("  REJECTED %s=%s: %s" % (knob, val, ex))
                     continue
                 history.append((dict(cand), ms))
                 if ms < best_ms * 0.98:

@@ -210,3 +210,47 @@ def test_bench_source_picker_matches_oracle(orc):
         np.testing.assert_array_equal(a, b)
         assert np.all(np.diff(off)[a] > 0) and len(set(a.tolist())) == a.size
     assert pick_sources(sparse, 5, 10).size == 3
+
+
+def test_autotuner_label_verifier(orc):
+    """The autotuner's own BFS-label check (product code, not the oracle)
+    accepts the oracle's labels and rejects each planted mistake: a depth
+    off by one, a predecessor that is not a neighbour, a predecessor on the
+    wrong level, a reachable vertex left unreached, an unreachable vertex
+    labelled, and a wrong source label."""
+    import numpy as np
+    from back40computing_amd.autotune import sorted_edge_keys, verify_bfs_labels
+    graphs = [orc.build_rmat_csr(10, 8, 0.57, 0.19, 0.19, 2), orc.build_lattice_csr(6, 1)[:2]]
+    for off, cols in graphs:
+        keys = sorted_edge_keys(off, cols)
+        src = int(orc.pick_sources(off, 4, 1)[0])
+        depth, pred, _ = orc.bfs(off, cols, src, with_pred=True)
+        assert verify_bfs_labels(off, cols, src, depth, pred, keys)
+        assert verify_bfs_labels(off, cols, src, depth, pred)  # builds its own keys
+        v = int(np.flatnonzero(depth == depth.max())[0])
+        bad = depth.copy(); bad[v] += 1
+        assert not verify_bfs_labels(off, cols, src, bad, pred, keys)
+        bad = depth.copy(); bad[v] -= 1
+        assert not verify_bfs_labels(off, cols, src, bad, pred, keys)
+        nbrs = set(cols[off[v]:off[v + 1]].tolist())
+        stranger = next(u for u in range(off.size - 1) if u not in nbrs and u != v and depth[u] == depth[v] - 1) \
+            if any(depth[u] == depth[v] - 1 and u not in nbrs for u in range(off.size - 1)) else None
+        if stranger is not None:
+            bp = pred.copy(); bp[v] = stranger
+            assert not verify_bfs_labels(off, cols, src, depth, bp, keys)
+        same_level = [u for u in nbrs if depth[u] == depth[v]]
+        if same_level:
+            bp = pred.copy(); bp[v] = same_level[0]
+            assert not verify_bfs_labels(off, cols, src, depth, bp, keys)
+        bad, bp = depth.copy(), pred.copy()
+        bad[v], bp[v] = -1, -1
+        assert not verify_bfs_labels(off, cols, src, bad, bp, keys)
+        bp = pred.copy(); bp[src] = -1
+        assert not verify_bfs_labels(off, cols, src, depth, bp, keys)
+    # An isolated vertex labelled as reached.
+    off = np.array([0, 1, 2, 2], np.int32)
+    cols = np.array([1, 0], np.int32)
+    depth = np.array([0, 1, -1], np.int32)
+    pred = np.array([-2, 0, -1], np.int32)
+    assert verify_bfs_labels(off, cols, 0, depth, pred)
+    assert not verify_bfs_labels(off, cols, 0, np.array([0, 1, 2], np.int32), np.array([-2, 0, 1], np.int32))
