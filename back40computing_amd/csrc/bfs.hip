@@ -94,12 +94,14 @@ __global__ __launch_bounds__(BFS_THREADS) void k_fused(BfsArgs a, int ph, int32_
 }
 
 __global__ __launch_bounds__(BFS_THREADS) void k_gbarrier_bench(GridBarrier bar, int iterations) {
-    gbarrier_bench_body(bar, iterations);
+    __shared__ uint32_t s_word;
+    gbarrier_bench_body(bar, iterations, &s_word);
 }
 
 __global__ __launch_bounds__(BFS_THREADS) void k_gbarrier_check(GridBarrier bar, uint32_t* data, int words,
                                                                 int iterations, unsigned long long* mismatches) {
-    gbarrier_check_body(bar, data, words, iterations, mismatches);
+    __shared__ uint32_t s_word;
+    gbarrier_check_body(bar, data, words, iterations, mismatches, &s_word);
 }
 
 }  // namespace b40c

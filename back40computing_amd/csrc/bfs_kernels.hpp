@@ -67,6 +67,14 @@ __device__ __forceinline__ uint32_t phase_input_len(const BfsArgs& a, int ph) {
     return n;
 }
 
+// A phase body's n_in argument: the input frontier's length when the caller
+// already has it (the persistent kernels get it with the grid barrier), or
+// this value to read it from the counters.
+constexpr uint32_t kLenFromCounters = 0xFFFFFFFFu;
+__device__ __forceinline__ uint32_t input_len(const BfsArgs& a, int ph, uint32_t n_in) {
+    return n_in == kLenFromCounters ? phase_input_len(a, ph) : n_in;
+}
+
 __device__ __forceinline__ void phase_begin(const BfsArgs& a, int ph, uint32_t n_in) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         st_relaxed(&a.ctr->qlen[(ph + 2) & 3], 0u);
@@ -184,8 +192,8 @@ __device__ __forceinline__ void td_finish(const BfsArgs& a, int ph, TdSink<VCAP>
 template <int VCAP>
 __device__ void td_phase(const BfsArgs& a, int ph, int32_t depth_next, uint32_t* next_bm, bool count_edges,
                          LdsQueueT<VCAP>& q, GatherSmem& g, unsigned long long* red, uint32_t nblocks,
-                         int32_t grid_th) {
-    const uint32_t n_in = phase_input_len(a, ph);
+                         int32_t grid_th, uint32_t n_in_arg = kLenFromCounters) {
+    const uint32_t n_in = input_len(a, ph, n_in_arg);
     phase_begin(a, ph, n_in);
     const int32_t* qin = a.qk[ph & 1];
     const GlobalQueue out = out_queue(a, ph);
@@ -212,8 +220,9 @@ __device__ void td_phase(const BfsArgs& a, int ph, int32_t depth_next, uint32_t*
 // labelled at contraction (depth = depth_here), then the survivors'
 // neighbour lists are expanded into the outgoing edge frontier
 // (contract_expand_atomic/cta.cuh:573-709).
-__device__ void ce_phase(const BfsArgs& a, int ph, int32_t depth_here, LevelShared& s) {
-    const uint32_t n_in = phase_input_len(a, ph);
+__device__ void ce_phase(const BfsArgs& a, int ph, int32_t depth_here, LevelShared& s,
+                         uint32_t n_in_arg = kLenFromCounters) {
+    const uint32_t n_in = input_len(a, ph, n_in_arg);
     phase_begin(a, ph, n_in);
     const int32_t* qk = a.qk[ph & 1];
     const int32_t* qv = a.qv[ph & 1];
@@ -248,8 +257,9 @@ __device__ void ce_phase(const BfsArgs& a, int ph, int32_t depth_here, LevelShar
 
 // Two-phase contract (edge frontier -> vertex frontier + labels),
 // two_phase/contract_atomic/cta.cuh:494-572.
-__device__ void tp_contract_phase(const BfsArgs& a, int ph, int32_t depth_here, LevelShared& s) {
-    const uint32_t n_in = phase_input_len(a, ph);
+__device__ void tp_contract_phase(const BfsArgs& a, int ph, int32_t depth_here, LevelShared& s,
+                                  uint32_t n_in_arg = kLenFromCounters) {
+    const uint32_t n_in = input_len(a, ph, n_in_arg);
     phase_begin(a, ph, n_in);
     const int32_t* qk = a.qk[ph & 1];
     const int32_t* qv = a.qv[ph & 1];
@@ -343,8 +353,8 @@ __device__ __forceinline__ void expand_tile_in_place(const BfsArgs& a, const Glo
 // Two-phase expand (vertex frontier -> edge frontier of (neighbour, pred)),
 // two_phase/expand_atomic/cta.cuh:606-719.  No culling: lists below the
 // grid threshold are expanded in place per tile, longer ones by k_heavy.
-__device__ void tp_expand_phase(const BfsArgs& a, int ph, LevelShared& s) {
-    const uint32_t n_in = phase_input_len(a, ph);
+__device__ void tp_expand_phase(const BfsArgs& a, int ph, LevelShared& s, uint32_t n_in_arg = kLenFromCounters) {
+    const uint32_t n_in = input_len(a, ph, n_in_arg);
     phase_begin(a, ph, n_in);
     const int32_t* qin = a.qk[ph & 1];
     const GlobalQueue out = out_queue(a, ph);
@@ -974,56 +984,68 @@ __device__ void fused_body(BfsArgs a, int ph, int32_t depth, int max_levels, uin
     if (a.wg_stats && threadIdx.x == 0) t0 = wg_clock();
     shared_init(s);
     const GridBarrier bar{a.bar_arrive, &a.ctr->bar_release, &a.ctr->bar_abort};
-    uint32_t epoch = ld_relaxed(bar.release);
+    uint32_t epoch = barrier_base_epoch(bar);
     if (blockIdx.x == 0 && threadIdx.x == 0) a.ctr->saturated = 0;
+    // Barrier payload: the length of the frontier queue in counter slot
+    // `slot` (clamped to 31 bits: a frontier never holds 2^31 entries, since
+    // E < 2^31) with bit 31 set when the grid tier has lists deferred by
+    // phase `heavy_ph` (heavy_ph < 0: not asked).  Workgroup 0 reads both
+    // after every workgroup has arrived.
+    auto payload_for = [&a](int slot, int heavy_ph) {
+        return [&a, slot, heavy_ph]() -> uint32_t {
+            uint32_t len = ld_relaxed(&a.ctr->qlen[slot & 3]);
+            if (len > 0x7FFFFFFFu) len = 0x7FFFFFFFu;
+            const bool deferred = heavy_ph >= 0 && __hip_atomic_load(&a.ctr->heavy_alloc[heavy_ph & 1], __ATOMIC_RELAXED,
+                                                                     __HIP_MEMORY_SCOPE_AGENT) != 0ull;
+            return len | (deferred ? 0x80000000u : 0u);
+        };
+    };
+    const bool grid_tier = a.grid_th != INT_MAX;
+    // Every thread reads the first level's length itself (written by the
+    // launch before this one); later lengths come with the barriers.
+    uint32_t n_in = ld_relaxed(&a.ctr->qlen[ph & 3]);
     int level = 0;
     for (; level < max_levels; ++level) {
-        if (threadIdx.x == 0) {
-            const uint32_t n_in = ld_relaxed(&a.ctr->qlen[ph & 3]);
-            s.ctl = (n_in == 0 || n_in > a.qcap[ph & 1]) ? 1u : (sat_limit && n_in > sat_limit) ? 2u : 0u;
-        }
-        __syncthreads();
-        const uint32_t stop = s.ctl;
-        __syncthreads();
-        if (stop) {
-            if (stop == 2 && blockIdx.x == 0 && threadIdx.x == 0) a.ctr->saturated = 1;
+        if (n_in == 0 || n_in > a.qcap[ph & 1]) break;
+        if (sat_limit && n_in > sat_limit) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) a.ctr->saturated = 1;
             break;
         }
         unsigned long long tb = 0;
         if (a.wg_stats && threadIdx.x == 0) tb = wg_clock();
         int heavy_ph = ph;          // phase whose deferred lists the grid tier takes
         int32_t heavy_depth = depth;  // depth label of what it discovers (top-down)
+        uint32_t word = 0;
         if (KIND == FK_TD) {
-            td_phase(a, ph, depth + 1, nullptr, false, s.q, s.g, s.red, gridDim.x, a.grid_th);
+            td_phase(a, ph, depth + 1, nullptr, false, s.q, s.g, s.red, gridDim.x, a.grid_th, n_in);
             heavy_depth = depth + 1;
             ph += 1;
         } else if (KIND == FK_CE) {
-            ce_phase(a, ph, depth, s);
+            ce_phase(a, ph, depth, s, n_in);
             ph += 1;
         } else {
-            tp_contract_phase(a, ph, depth, s);
+            tp_contract_phase(a, ph, depth, s, n_in);
             if (a.wg_stats && threadIdx.x == 0) busy += wg_clock() - tb;
-            if (!grid_sync(bar, ++epoch)) break;
+            if (!grid_sync(bar, ++epoch, payload_for(ph + 1, -1), &s.ctl, word)) break;
             if (a.wg_stats && threadIdx.x == 0) tb = wg_clock();
-            tp_expand_phase(a, ph + 1, s);
+            // An overflowed vertex queue counts as empty (as phase_input_len
+            // has it); the overflow flag ends the search.
+            uint32_t n_mid = word & 0x7FFFFFFFu;
+            if (n_mid > a.qcap[(ph + 1) & 1]) n_mid = 0;
+            tp_expand_phase(a, ph + 1, s, n_mid);
             heavy_ph = ph + 1;
             ph += 2;
         }
         depth += 1;
         if (a.wg_stats && threadIdx.x == 0) busy += wg_clock() - tb;
-        if (!grid_sync(bar, ++epoch)) break;
-        if (a.grid_th != INT_MAX) {
-            if (threadIdx.x == 0)
-                s.ctl = __hip_atomic_load(&a.ctr->heavy_alloc[heavy_ph & 1], __ATOMIC_RELAXED,
-                                          __HIP_MEMORY_SCOPE_AGENT) != 0ull;
-            __syncthreads();
-            const bool deferred = s.ctl != 0;
-            __syncthreads();
-            if (deferred) {
-                heavy_body<FusedHeavy<KIND>::kind>(a, heavy_ph, heavy_depth, 0, hs);
-                if (!grid_sync(bar, ++epoch)) break;
-            }
+        if (!grid_sync(bar, ++epoch, payload_for(ph, grid_tier ? heavy_ph : -1), &s.ctl, word)) break;
+        if (word & 0x80000000u) {
+            // Lists this level deferred: the whole grid takes them in
+            // chunks, appending to the same next frontier.
+            heavy_body<FusedHeavy<KIND>::kind>(a, heavy_ph, heavy_depth, 0, hs);
+            if (!grid_sync(bar, ++epoch, payload_for(ph, -1), &s.ctl, word)) break;
         }
+        n_in = word & 0x7FFFFFFFu;
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         a.ctr->levels_run = (uint32_t)level;
@@ -1037,10 +1059,10 @@ __device__ void fused_body(BfsArgs a, int ph, int32_t depth, int max_levels, uin
 
 // Global-barrier benchmark (test/gbarrier/test_gbarrier.cu:43-50): the
 // kernel does nothing but pass `iterations` barriers.
-__device__ void gbarrier_bench_body(const GridBarrier& bar, int iterations) {
-    uint32_t epoch = ld_relaxed(bar.release);
+__device__ void gbarrier_bench_body(const GridBarrier& bar, int iterations, uint32_t* s_word) {
+    uint32_t epoch = barrier_base_epoch(bar);
     for (int i = 0; i < iterations; ++i)
-        if (!grid_sync(bar, ++epoch)) break;
+        if (!grid_sync(bar, ++epoch, s_word)) break;
 }
 
 // Cross-workgroup visibility check of the grid barrier: each workgroup writes
@@ -1056,20 +1078,26 @@ __device__ __forceinline__ uint32_t gb_pattern(uint32_t it, uint32_t b, uint32_t
 }
 
 __device__ void gbarrier_check_body(const GridBarrier& bar, uint32_t* data, int words, int iterations,
-                                    unsigned long long* mismatches) {
+                                    unsigned long long* mismatches, uint32_t* s_word) {
     const uint32_t G = gridDim.x, b = blockIdx.x;
-    uint32_t epoch = ld_relaxed(bar.release);
+    uint32_t epoch = barrier_base_epoch(bar);
     unsigned long long bad = 0;
     for (int it = 0; it < iterations; ++it) {
         for (int j = threadIdx.x; j < words; j += BFS_THREADS) data[(size_t)b * words + j] = gb_pattern(it, b, j);
-        if (!grid_sync(bar, ++epoch)) break;
+        // The payload is the iteration number as workgroup 0 saw it in a
+        // word every workgroup wrote before arriving: all must agree.
+        uint32_t seen = 0;
+        if (!grid_sync(bar, ++epoch, [&] { return data[(size_t)(G - 1) * words] ^ gb_pattern(it, G - 1, 0) ^ (uint32_t)it; },
+                       s_word, seen))
+            break;
+        bad += (seen != (uint32_t)it) ? 1u : 0u;
         const uint32_t others[3] = {(b + 1) % G, (b + G / 2 + 1) % G, (b + G - 1) % G};
         for (int k = 0; k < 3; ++k) {
             const uint32_t o = others[k];
             for (int j = threadIdx.x; j < words; j += BFS_THREADS)
                 bad += (data[(size_t)o * words + j] != gb_pattern(it, o, j)) ? 1u : 0u;
         }
-        if (!grid_sync(bar, ++epoch)) break;
+        if (!grid_sync(bar, ++epoch, s_word)) break;
     }
     if (bad) atomic_add_u64(mismatches, bad);
 }

@@ -50,8 +50,10 @@ struct alignas(256) DevCounters {
     unsigned long long queued;   // total elements taken from input frontiers
     unsigned long long next_edges;  // degree sum of the vertices discovered this level
     unsigned long long next_edges_prev;
-    uint32_t bar_pad;
-    uint32_t bar_release;        // grid barrier: epoch of the last completed barrier (never reset by a search)
+    // Grid barrier release word: low 32 bits the epoch of the last completed
+    // barrier (never reset by a search), high 32 bits the word workgroup 0
+    // published with it.
+    unsigned long long bar_release;
     uint32_t bar_abort;          // set when a barrier wait timed out
     uint32_t corrupt;            // set when a frontier held an id outside [0, n)
     unsigned long long level_edges[4];  // rotating per-level degree sums (direction switch)
@@ -147,12 +149,19 @@ __device__ __forceinline__ int wg_exclusive_scan(int x, int& total, int* s_wave)
 // word, so there is no same-address atomic traffic; a barrier costs about
 // three agent-scope round trips (arrival store, master poll, release poll).
 //
+// The release word is 64 bits: the epoch in the low half and, in the high
+// half, a word workgroup 0 reads after every workgroup has arrived (the next
+// frontier's length, say).  Waiters get it with the same load that ends
+// their wait, so every workgroup starts the next phase with the same value
+// and without another memory round trip.
+//
 // Epochs grow monotonically over the lifetime of the barrier memory: a
-// kernel reads *release once at its start (every workgroup reads the same
-// value, since no barrier can complete before all have arrived) and passes
-// base + 1, base + 2, ... to successive barriers.  Arrival words therefore
-// only ever hold epochs up to the current one, and equality means "arrived".
-// Agent scope makes the level's writes visible across all XCDs' L2s.
+// kernel reads the release word's epoch once at its start (every workgroup
+// reads the same value, since no barrier can complete before all have
+// arrived) and passes base + 1, base + 2, ... to successive barriers.
+// Arrival words therefore only ever hold epochs up to the current one, and
+// equality means "arrived".  Agent scope makes the level's writes visible
+// across all XCDs' L2s.
 //
 // The wait is bounded: after kBarrierSpinLimit polls a workgroup raises
 // `abort_flag` and leaves, and waiters leave as soon as they see the flag, so
@@ -165,12 +174,22 @@ __device__ __forceinline__ int wg_exclusive_scan(int x, int& total, int* s_wave)
 constexpr uint32_t kBarrierSpinLimit = 1u << 22;
 
 struct GridBarrier {
-    uint32_t* arrive;      // [grid] arrival epoch of each workgroup (word 0 unused)
-    uint32_t* release;     // epoch of the last completed barrier
-    uint32_t* abort_flag;  // nonzero once a wait ran out of polls
+    uint32_t* arrive;              // [grid] arrival epoch of each workgroup (word 0 unused)
+    unsigned long long* release;   // (payload << 32) | epoch of the last completed barrier
+    uint32_t* abort_flag;          // nonzero once a wait ran out of polls
 };
 
-__device__ __forceinline__ bool grid_sync(const GridBarrier& b, uint32_t epoch) {
+__device__ __forceinline__ uint32_t barrier_base_epoch(const GridBarrier& b) {
+    return (uint32_t)__hip_atomic_load(b.release, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// make_payload() runs in thread 0 of workgroup 0 after every workgroup has
+// arrived (and after an acquire fence); its value reaches every thread of
+// every workgroup in `payload`.  s_word: one LDS word of the calling
+// workgroup, used to broadcast the payload.
+template <typename PayloadFn>
+__device__ __forceinline__ bool grid_sync(const GridBarrier& b, uint32_t epoch, PayloadFn make_payload,
+                                          uint32_t* s_word, uint32_t& payload) {
     __syncthreads();
     bool aborted = false;
     const uint32_t nb = gridDim.x;
@@ -189,12 +208,17 @@ __device__ __forceinline__ bool grid_sync(const GridBarrier& b, uint32_t epoch) 
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         aborted = __syncthreads_or(aborted) != 0;
-        if (threadIdx.x == 0 && !aborted)
-            __hip_atomic_store(b.release, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (threadIdx.x == 0 && !aborted) {
+            const uint32_t p = make_payload();
+            *s_word = p;
+            __hip_atomic_store(b.release, ((unsigned long long)p << 32) | epoch, __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        }
     } else if (threadIdx.x == 0) {
         __hip_atomic_store(&b.arrive[blockIdx.x], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         uint32_t spins = 0;
-        while (ld_relaxed(b.release) != epoch) {
+        unsigned long long r;
+        while ((uint32_t)(r = __hip_atomic_load(b.release, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != epoch) {
             if (ld_relaxed(b.abort_flag) || ++spins > kBarrierSpinLimit) {
                 st_relaxed(b.abort_flag, 1u);
                 aborted = true;
@@ -203,8 +227,18 @@ __device__ __forceinline__ bool grid_sync(const GridBarrier& b, uint32_t epoch) 
             __builtin_amdgcn_s_sleep(1);
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        *s_word = (uint32_t)(r >> 32);
     }
-    return __syncthreads_or(aborted) == 0;
+    const bool ok = __syncthreads_or(aborted) == 0;
+    payload = *s_word;
+    __syncthreads();  // s_word may be rewritten by the next barrier
+    return ok;
+}
+
+// Barrier without a payload.
+__device__ __forceinline__ bool grid_sync(const GridBarrier& b, uint32_t epoch, uint32_t* s_word) {
+    uint32_t unused = 0;
+    return grid_sync(b, epoch, [] { return 0u; }, s_word, unused);
 }
 
 // ---------------------------------------------------------------------------

@@ -146,3 +146,15 @@ def test_replay_forced_overflow(replay_exe, workdir, rmat10, sm):
     off, cols, src = rmat10
     status, _, _, _ = replay(replay_exe, workdir, off, cols, src, *sm, grid=2, queue_sizing=0.01)
     assert status == 1
+
+
+@pytest.mark.parametrize("grid", [2, 5])
+def test_replay_grid_barrier(replay_exe, grid):
+    """The grid barrier's visibility check (words written before a barrier
+    are seen after it, and the payload workgroup 0 publishes is the same in
+    every workgroup), over two launches that continue one epoch sequence."""
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
+    r = subprocess.run([replay_exe, "--gbarrier", str(grid), "30"], capture_output=True, text=True, env=env,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr[-3000:]
+    assert r.stdout.split() == ["mismatches", "0", "aborted", "0", "epoch", str(2 * 30 * 2)]
