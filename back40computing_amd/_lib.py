@@ -140,6 +140,7 @@ SIGNATURES = {
     "b40c_radix_sort_u64_noalloc": (_i32, [_vp, _vp, _i64, _i32, _i32, _p_i32]),
     "b40c_sort_enactor_create": (_i32, [_pp]),
     "b40c_sort_enactor_sort": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, _i64, _i32, _i32, _p_i32]),
+    "b40c_sort_enactor_set_policy": (_i32, [_vp, _i32, _i32]),
     "b40c_sort_enactor_destroy": (None, [_vp]),
     "b40c_device_alloc": (_i32, [_pp, _i64]),
     "b40c_device_free": (_i32, [_vp]),
@@ -168,6 +169,7 @@ SIGNATURES = {
     "b40c_msbfs_create": (_i32, [_vp, _i64, _i32, _pp]),
     "b40c_msbfs_set_tuning": (_i32, [_vp, _i32, _i32, _i32]),
     "b40c_msbfs_set_admission": (_i32, [_vp, _i32]),
+    "b40c_msbfs_set_levels_per_step": (_i32, [_vp, _i32]),
     "b40c_msbfs_submit": (_i32, [_vp, _i32, _i32, _p_i32]),
     "b40c_msbfs_step": (_i32, [_vp, _p_i32, _p_i32]),
     "b40c_msbfs_run": (_i32, [_vp]),

@@ -43,18 +43,19 @@ SPACE = {
     "levels_per_sync": [2, 4, 6, 7, 8, 12, 16, 32, 64],
     "saturation_quit": [64, 256, 512, 2048],
     "bottom_up_grid_size": [512, 1024, 1280, 1536, 2048, 4096],
+    "persistent_wgs_per_cu": [1, 2, 3, 4, 6, 8],
 }
 
 # Knobs that matter per strategy.
 KNOBS = {
-    _lib.EXPAND_CONTRACT: ["grid_gather_threshold", "cta_gather_threshold", "wave_gather_threshold",
-                           "grid_size", "levels_per_sync"],
-    _lib.CONTRACT_EXPAND: ["grid_gather_threshold", "cta_gather_threshold", "wave_gather_threshold",
-                           "grid_size", "levels_per_sync"],
-    _lib.TWO_PHASE: ["grid_gather_threshold", "cta_gather_threshold", "wave_gather_threshold",
-                     "grid_size", "levels_per_sync"],
-    _lib.HYBRID: ["saturation_quit", "grid_gather_threshold", "cta_gather_threshold", "wave_gather_threshold",
-                  "grid_size"],
+    _lib.EXPAND_CONTRACT: ["persistent_wgs_per_cu", "grid_gather_threshold", "cta_gather_threshold",
+                           "wave_gather_threshold", "grid_size", "levels_per_sync"],
+    _lib.CONTRACT_EXPAND: ["persistent_wgs_per_cu", "grid_gather_threshold", "cta_gather_threshold",
+                           "wave_gather_threshold", "grid_size", "levels_per_sync"],
+    _lib.TWO_PHASE: ["persistent_wgs_per_cu", "grid_gather_threshold", "cta_gather_threshold",
+                     "wave_gather_threshold", "grid_size", "levels_per_sync"],
+    _lib.HYBRID: ["saturation_quit", "persistent_wgs_per_cu", "grid_gather_threshold", "cta_gather_threshold",
+                  "wave_gather_threshold", "grid_size"],
     _lib.DIRECTION_OPTIMIZING: ["bottom_up_grid_size", "do_alpha", "do_beta", "grid_gather_threshold",
                                 "grid_size", "levels_per_sync", "wave_gather_threshold"],
 }
@@ -271,6 +272,103 @@ def tune_and_record(graph, strategy, sources, path=TABLE_PATH, log=None, **kw):
     return best, ms
 
 
+# Radix-sort policies (b40c_sort_enactor_set_policy): keys per thread of a
+# downsweep tile and workgroups per CU, the run-time form of the reference's
+# per-architecture sort policies (radix_sort/autotuned_policy.cuh:190-230).
+SORT_SPACE = [(items, wgs) for items in (8, 16) for wgs in (1, 2, 3, 4, 6, 8)]
+
+SORT_KEY_DTYPES = {"u32": np.uint32, "i32": np.int32, "f32": np.float32, "u64": np.uint64, "i64": np.int64,
+                   "f64": np.float64}
+
+
+def _sort_key(arch, key_name, with_values, n):
+    return "%s/sort/%s%s/n%d" % (arch.split(":")[0], key_name, "_pairs" if with_values else "",
+                                 int(round(math.log2(max(2, n)))))
+
+
+def lookup_sort(arch, key_name, with_values, n, path=TABLE_PATH):
+    """(items_per_thread, wgs_per_cu) recorded for this sort class, or None."""
+    ent = load_table(path).get(_sort_key(arch, key_name, with_values, n))
+    return (ent["items_per_thread"], ent["wgs_per_cu"]) if ent else None
+
+
+def _random_keys(rng, dt, n):
+    if np.issubdtype(dt, np.floating):
+        return rng.standard_normal(n).astype(dt)
+    info = np.iinfo(dt)
+    return rng.integers(info.min, info.max, n, dtype=dt, endpoint=True)
+
+
+def tune_sort(n, key_name="u64", with_values=False, repeats=3, seed=5, log=None, space=None):
+    """Times every policy of SORT_SPACE on n random keys (of the key type, with
+    uint32 values if asked) with a sort enactor, best of `repeats` per policy.
+    Every policy's output is compared with numpy's stable sort of the same
+    input (keys, and the values carried with them) before its time counts.
+    Returns (best policy, its ms, history)."""
+    from . import radix_sort as rs
+    dt = np.dtype(SORT_KEY_DTYPES[key_name])
+    rng = np.random.default_rng(seed)
+    keys = _random_keys(rng, dt, n)
+    vals = np.arange(n, dtype=np.uint32) if with_values else None
+    # Random normals hold no NaN and no signed zero, where numpy's order and
+    # the bitwise key order could differ.
+    order = np.argsort(keys, kind="stable")
+    want_keys = keys[order]
+    en = rs.Enactor()
+    db = rs.DoubleBuffer(dt, n, with_values=with_values)
+    timer = _Timer()
+    best, best_ms, history = None, None, []
+    try:
+        for items, wgs in (space or SORT_SPACE):
+            en.set_policy(items, wgs)
+            ms = None
+            for _ in range(repeats):
+                db.selector = 0
+                db.d_keys[0].upload(keys)
+                if with_values:
+                    db.d_values[0].upload(vals)
+                t0 = time.perf_counter()
+                check(en.Sort(db, n))
+                dt_ms = (time.perf_counter() - t0) * 1e3
+                ms = dt_ms if ms is None else min(ms, dt_ms)
+            got = db.d_keys[db.selector].download(dt, n)
+            ok = np.array_equal(got.view(np.uint8), want_keys.view(np.uint8))
+            if with_values:
+                ok = ok and np.array_equal(db.d_values[db.selector].download(np.uint32, n), order.astype(np.uint32))
+            if not ok:
+                history.append(((items, wgs), None))
+                if log:
+                    log("  REJECTED sort policy items=%d wgs_per_cu=%d: output differs from numpy" % (items, wgs))
+                continue
+            history.append(((items, wgs), ms))
+            if log:
+                log("  sort %s%s n=%d items=%d wgs_per_cu=%d: %.3f ms" % (key_name, "+v" if with_values else "", n,
+                                                                        items, wgs, ms))
+            if best_ms is None or ms < best_ms:
+                best, best_ms = (items, wgs), ms
+    finally:
+        for a in db.d_keys + (db.d_values or []):
+            a.free()
+        en.close()
+    return best, best_ms, history
+
+
+def tune_sort_and_record(n, key_name="u64", with_values=False, path=TABLE_PATH, log=None, **kw):
+    from . import bfs
+    arch = bfs.device_info()["arch"]
+    best, ms, hist = tune_sort(n, key_name, with_values, log=log, **kw)
+    if best is None:
+        raise AssertionError("no sort policy reproduced numpy's order")
+    table = load_table(path)
+    table[_sort_key(arch, key_name, with_values, n)] = {
+        "items_per_thread": best[0], "wgs_per_cu": best[1], "ms": round(ms, 4), "n": n,
+        "evaluated": len(hist), "all_ms": {"%d/%d" % p: (round(t, 4) if t else None) for p, t in hist},
+    }
+    with open(path, "w") as f:
+        json.dump(table, f, indent=1, sort_keys=True)
+    return best, ms
+
+
 def main(argv=None):
     """Tune strategies on a synthetic RMAT or lattice graph and write the table.
 
@@ -289,10 +387,19 @@ def main(argv=None):
     ap.add_argument("--sources", type=int, default=8)
     ap.add_argument("--rounds", type=int, default=2)
     ap.add_argument("--out", default=TABLE_PATH)
+    ap.add_argument("--sort", default=None,
+                    help="tune the radix sort instead: KEY[,pairs] (KEY in u32 i32 f32 u64 i64 f64), e.g. u64")
+    ap.add_argument("--sort-n", type=int, default=1 << 27, help="keys per sort when tuning the radix sort")
     args = ap.parse_args(argv)
 
     def log(*a):
         print(*a, file=sys.stderr, flush=True)
+
+    if args.sort:
+        parts = args.sort.split(",")
+        best, ms = tune_sort_and_record(args.sort_n, parts[0], "pairs" in parts[1:], path=args.out, log=log)
+        log("sort %s n=%d best items=%d wgs_per_cu=%d: %.3f ms" % (args.sort, args.sort_n, best[0], best[1], ms))
+        return 0
 
     if args.graph == "rmat":
         g = bfs.Graph.rmat(args.scale, args.edge_factor, 0.57, 0.19, 0.19, 0)

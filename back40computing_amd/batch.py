@@ -60,6 +60,11 @@ class MultiSourceBFS:
         """Levels between admissions while queries run (1: every boundary)."""
         check(lib().b40c_msbfs_set_admission(self._h, int(cohort_gap)))
 
+    def set_levels_per_step(self, levels):
+        """Levels per step: >= 1 replays a captured hipGraph of that many
+        levels with one synchronisation; 0 launches one level per step."""
+        check(lib().b40c_msbfs_set_levels_per_step(self._h, int(levels)))
+
     def submit(self, graph, src):
         q = ctypes.c_int32()
         check(lib().b40c_msbfs_submit(self._h, int(graph), int(src), ctypes.byref(q)))

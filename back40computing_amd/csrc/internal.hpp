@@ -48,14 +48,24 @@ struct SortScratch {
     uint32_t* trivial = nullptr;
     size_t words = 0;
 };
+// A sort's launch policy (the reference's per-architecture KernelPolicy /
+// AutotunedPolicy, radix_sort/autotuned_policy.cuh:190-230, chosen at run
+// time): keys per thread of a downsweep tile (8 or 16; a tile is 256 x that)
+// and workgroups per CU (1-8).  Zero fields take the size-based default.
+struct SortPolicy {
+    int items = 0;
+    int wgs_per_cu = 0;
+};
+// Policy with its zero fields filled in for n keys of key_bytes bytes.
+SortPolicy resolve_sort_policy(SortPolicy p, int64_t n, int key_bytes);
 // Words of scratch a sort of n keys needs on a device with num_cus CUs.
-size_t sort_scratch_words(int64_t n, int num_cus);
+size_t sort_scratch_words(int64_t n, int num_cus, const SortPolicy& p);
 SortScratch sort_scratch_view(uint32_t* words, size_t count);
 // Launches the passes (no allocation, no synchronization); the selector is
 // left in *sc.sel.  `launched` is false when there was nothing to sort.
 b40c_status radix_sort_async(int32_t key_type, void* k0, void* k1, uint32_t* v0, uint32_t* v1, int64_t n,
-                             int begin_bit, int end_bit, int num_cus, const SortScratch& sc, hipStream_t stream,
-                             bool& launched);
+                             int begin_bit, int end_bit, int num_cus, const SortPolicy& policy,
+                             const SortScratch& sc, hipStream_t stream, bool& launched);
 b40c_status radix_sort_u64(uint64_t* k0, uint64_t* k1, int64_t n, int begin_bit, int end_bit, int32_t* selector,
                            hipStream_t stream);
 b40c_status radix_sort_pairs_u32(uint32_t* k0, uint32_t* k1, uint32_t* v0, uint32_t* v1, int64_t n, int begin_bit,

@@ -63,7 +63,6 @@ struct MsArgs {
     unsigned long long* front[3];
     unsigned long long* active;        // [G] slots in use per graph
     unsigned long long* bu;            // [G] slots in bottom-up mode this level
-    unsigned long long* newbits;       // [G] slots that discovered vertices in the last level
     uint32_t* cnt_n;                   // [G*64] vertices discovered in the last level, per slot
     unsigned long long* cnt_m;         // [G*64] their degree sum
     uint32_t* nf_prev;                 // [G*64] the level before's count
@@ -77,7 +76,8 @@ struct MsArgs {
     uint32_t* fbm[2];                  // [ceil(n/32)] by level parity: vertices carrying any bit in front[]
     int2* bu_list;                     // [2][G] by level parity: (graph, first tile) of graphs with bottom-up slots
     MsCounters* hctr;                  // device address of the pinned host mirror (overflow, corrupt)
-    unsigned long long* hnewbits;      // device address of the pinned host copy of newbits
+    int32_t* hdone;                    // [Q] device address of pinned host words: levels of a finished query
+    int32_t* dph;                      // captured level loops: phase of the replay's first level (else null)
     int2* heavy;                       // grid-tier entries (vertex, first chunk)
     int32_t* chunk_map;                // chunk -> entry
     uint32_t qcap;
@@ -98,6 +98,13 @@ __device__ __forceinline__ unsigned long long atomic_or_u64(unsigned long long* 
 }
 __device__ __forceinline__ void atomic_and_u64(unsigned long long* p, unsigned long long v) {
     (void)__hip_atomic_fetch_and(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// A level kernel's phase: the launch argument itself, or, in a captured
+// level loop, the argument (the level's index within the replay) plus the
+// replay's first phase, which k_ms_advance moves on at the end of a replay.
+__device__ __forceinline__ int ms_phase(const MsArgs& a, int ph) {
+    return a.dph ? (int)ld_relaxed((const uint32_t*)a.dph) + ph : ph;
 }
 
 // 64x64 bit-matrix transpose across a wave: on return, bit i of lane b's
@@ -216,6 +223,7 @@ __device__ __forceinline__ void ms_defer_heavy(const MsArgs& a, int ph, int32_t 
 // Vertices that gain bits are appended once (the first OR into next[v]
 // enqueues) to the next frontier queue.
 __global__ __launch_bounds__(BFS_THREADS) void k_ms_push(MsArgs a, int ph) {
+    ph = ms_phase(a, ph);
     __shared__ MsShared s;
     uint32_t n_f = ld_relaxed(&a.ctr->qlen[ph & 3]);
     if (n_f > a.qcap) n_f = 0;
@@ -320,6 +328,7 @@ __device__ __forceinline__ unsigned long long group_excl_or(unsigned long long x
 }
 
 __global__ __launch_bounds__(BFS_THREADS) void k_ms_pull(MsArgs a, int ph) {
+    ph = ms_phase(a, ph);
     __shared__ MsPullShared s;
     uint32_t n_f = ld_relaxed(&a.ctr->qlen[ph & 3]);
     if (n_f > a.qcap) n_f = 0;
@@ -341,7 +350,6 @@ __global__ __launch_bounds__(BFS_THREADS) void k_ms_pull(MsArgs a, int ph) {
     unsigned long long* next = a.front[(ph + 1) % 3];
     const uint32_t* fbm = a.fbm[ph & 1];
     const int tid = threadIdx.x;
-    const int lane = lane_id();
     const int grp = tid / MS_GROUP, sub = tid % MS_GROUP;
     for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
         int32_t base, end, tg = -1;
@@ -444,6 +452,7 @@ __global__ __launch_bounds__(BFS_THREADS) void k_ms_pull(MsArgs a, int ph) {
 // per thread.  Each chunk is checked against the entry it names, so a stale
 // map word never indexes outside a list.
 __global__ __launch_bounds__(BFS_THREADS) void k_ms_heavy(MsArgs a, int ph) {
+    ph = ms_phase(a, ph);
     __shared__ MsShared s;
     const unsigned long long alloc =
         __hip_atomic_load(&a.ctr->heavy_alloc[ph & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -485,6 +494,7 @@ __global__ __launch_bounds__(BFS_THREADS) void k_ms_heavy(MsArgs a, int ph) {
 // it, transposes the lanes' bit masks so that lane b holds the set of lanes
 // carrying bit b.  Running totals are kept while the graph stays the same.
 __global__ __launch_bounds__(BFS_THREADS) void k_ms_count(MsArgs a, int ph) {
+    ph = ms_phase(a, ph);
     uint32_t n_q = ld_relaxed(&a.ctr->qlen[(ph + 1) & 3]);
     // Frontier bitmap of level ph + 1: set entry by entry from the queue while
     // it is short (the array is all zero: its bits were those of level ph - 1,
@@ -571,9 +581,15 @@ __global__ __launch_bounds__(BFS_THREADS) void k_ms_count(MsArgs a, int ph) {
 // direction for level ph + 1: top-down -> bottom-up when the new frontier's
 // degree sum m_f exceeds m_u / alpha (m_u = edges of the graph not yet
 // reached by this query), bottom-up -> top-down when the frontier has shrunk
-// below V_g / beta.  Also publishes the "found anything" mask per graph and
-// consumes (zeroes) the counts.
+// below V_g / beta.  Consumes (zeroes) the counts.
+//
+// Retirement happens here too: a slot in use whose level discovered nothing
+// has finished.  Its active bit is cleared (later levels, also those of the
+// same captured replay, skip it) and the query's level count is written to
+// the pinned host word hdone[q]; the host frees the slot and its labels at
+// its next level boundary (k_ms_release clears the slot's visited bits).
 __global__ __launch_bounds__(BFS_THREADS) void k_ms_direction(MsArgs a, int ph) {
+    ph = ms_phase(a, ph);
     const int32_t g = (int32_t)(blockIdx.x * BFS_WAVES + threadIdx.x / kWave);
     if (blockIdx.x == 0 && threadIdx.x == 0) st_relaxed(&a.ctr->any_bu[ph & 1], 0u);
     if (g >= a.G) return;
@@ -601,11 +617,15 @@ __global__ __launch_bounds__(BFS_THREADS) void k_ms_direction(MsArgs a, int ph) 
     }
     a.nf_prev[k] = n;
     const unsigned long long bum = __ballot(to_bu);
-    const unsigned long long found = __ballot(occupied && n != 0u);
+    const bool finished = occupied && n == 0u;
+    const unsigned long long done = __ballot(finished);
+    if (finished) {
+        const int32_t q = a.bitq[k];
+        if (q >= 0) a.hdone[q] = ph + 1 - a.qstart[q];
+    }
     if (b == 0) {
         a.bu[g] = bum;
-        a.newbits[g] = found;
-        a.hnewbits[g] = found;
+        if (done) atomic_and_u64(&a.active[g], ~done);
         if (bum) {
             // List the graph for the next level's pull: one 64-bit atomic
             // gives both its table entry and its first tile, so the table
@@ -619,12 +639,25 @@ __global__ __launch_bounds__(BFS_THREADS) void k_ms_direction(MsArgs a, int ph) 
             if (e < (uint32_t)a.G) a.bu_list[(size_t)((ph + 1) & 1) * a.G + e] = make_int2(g, (int)(uint32_t)t);
         }
     }
-    // The step's last launch: publish the error flags with the masks (the
-    // host reads both after one stream synchronisation, no copies).
+    // The level's last launch: publish the error flags beside the finished
+    // queries' words (the host reads both after one stream synchronisation,
+    // no copies).
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         a.hctr->overflow = ld_relaxed(&a.ctr->overflow);
         a.hctr->corrupt = ld_relaxed(&a.ctr->corrupt);
     }
+}
+
+// Last node of a captured level loop: moves the replay's first phase on by
+// the `levels` levels the replay ran.
+__global__ void k_ms_advance(MsArgs a, int levels) {
+    if (threadIdx.x == 0) st_relaxed((uint32_t*)a.dph, ld_relaxed((const uint32_t*)a.dph) + (uint32_t)levels);
+}
+
+// Sets the device phase word outside a captured loop (when launches and
+// replays are mixed on one engine).
+__global__ void k_ms_set_phase(MsArgs a, int ph) {
+    if (threadIdx.x == 0) st_relaxed((uint32_t*)a.dph, (uint32_t)ph);
 }
 
 
@@ -793,8 +826,8 @@ struct b40c_msbfs {
     hipStream_t stream = nullptr;
     int32_t n = 0, G = 0;
     int64_t arena_elems = 0;  // per label array (depth and pred each)
-    b40c::DeviceBuffer visited, front[3], active, bu, newbits, cnt_n, cnt_m, nf_prev, m_seen, bitq, lbase, qstart, depth,
-        pred, queue[3], counters;
+    b40c::DeviceBuffer visited, front[3], active, bu, cnt_n, cnt_m, nf_prev, m_seen, bitq, lbase, qstart, depth,
+        pred, queue[3], counters, dph;
     b40c::DeviceBuffer seed_buf, rel_buf, heavy, chunk_map, fbm, bu_list;
     b40c::MsSeed* h_seeds = nullptr;  // pinned staging
     b40c::MsRelease* h_rel = nullptr;  // pinned staging
@@ -814,9 +847,22 @@ struct b40c_msbfs {
     int32_t last_admit_ph = -1000;
     b40c::MsCounters* h_ctr = nullptr;      // pinned, written by k_ms_direction (overflow, corrupt)
     b40c::MsCounters* h_ctr_dev = nullptr;  // its device address
-    unsigned long long* h_newbits_dev = nullptr;
-    unsigned long long* h_newbits = nullptr;  // pinned [G], written by k_ms_direction
+    int32_t* h_done = nullptr;      // pinned [qmax]: levels of a query once finished (0 while running)
+    int32_t* h_done_dev = nullptr;  // its device address
     int32_t qmax = 0;
+    // Captured level loop (levels_per_replay > 0): the five level launches of
+    // `levels_per_replay` levels and k_ms_advance, replayed as one graph.
+    int32_t levels_per_replay = 0;
+    hipGraph_t loop_graph = nullptr;
+    hipGraphExec_t loop_exec = nullptr;
+    int32_t loop_levels = 0, loop_grid = 0, loop_alpha = 0, loop_beta = 0;
+    int32_t dev_ph = 0;  // value of the device phase word (dph)
+    void release_loop() {
+        if (loop_exec) (void)hipGraphExecDestroy(loop_exec);
+        if (loop_graph) (void)hipGraphDestroy(loop_graph);
+        loop_exec = nullptr;
+        loop_graph = nullptr;
+    }
 };
 
 namespace b40c {
@@ -833,7 +879,6 @@ MsArgs ms_args(b40c_msbfs* m) {
     for (int i = 0; i < 3; ++i) a.front[i] = m->front[i].as<unsigned long long>();
     a.active = m->active.as<unsigned long long>();
     a.bu = m->bu.as<unsigned long long>();
-    a.newbits = m->newbits.as<unsigned long long>();
     a.cnt_n = m->cnt_n.as<uint32_t>();
     a.cnt_m = m->cnt_m.as<unsigned long long>();
     a.nf_prev = m->nf_prev.as<uint32_t>();
@@ -850,7 +895,8 @@ MsArgs ms_args(b40c_msbfs* m) {
     a.fbm[1] = m->fbm.as<uint32_t>() + words;
     a.bu_list = m->bu_list.as<int2>();
     a.hctr = m->h_ctr_dev;
-    a.hnewbits = m->h_newbits_dev;
+    a.hdone = m->h_done_dev;
+    a.dph = nullptr;
     a.heavy = m->heavy.as<int2>();
     a.chunk_map = m->chunk_map.as<int32_t>();
     a.heavy_cap = m->heavy_cap;
@@ -899,6 +945,10 @@ b40c_status ms_admit(b40c_msbfs* m) {
         Q.lbase = lb;
         Q.qstart = m->ph;
         Q.state = 1;
+        // The id may have been used before: its done word restarts at 0
+        // before the launch that admits it, so the device's later write is
+        // the only one that can follow.
+        m->h_done[q] = 0;
         m->h_active[g] |= 1ull << b;
         max_vg = std::max(max_vg, vg);
         seeds.push_back(MsSeed{q, m->mg->vbase[g] + Q.src, b, m->ph, (long long)lb});
@@ -933,18 +983,20 @@ b40c_status ms_admit(b40c_msbfs* m) {
     return B40C_OK;
 }
 
-// Retires queries whose slot found nothing new in the level just run.
+// Retires the queries the device marked finished (k_ms_direction wrote their
+// level counts) in the levels just run.
 b40c_status ms_retire(b40c_msbfs* m) {
     std::vector<int32_t> keep;
     std::map<int32_t, unsigned long long> masks;  // graph -> retired slots
     for (int32_t q : m->running) {
         MsQuery& Q = m->queries[q];
         const unsigned long long bit = 1ull << Q.bit;
-        Q.levels = m->ph + 1 - Q.qstart;
-        if (m->h_newbits[Q.graph] & bit) {
+        const int32_t levels = __atomic_load_n(&m->h_done[q], __ATOMIC_ACQUIRE);
+        if (levels <= 0) {
             keep.push_back(q);
             continue;
         }
+        Q.levels = levels;
         Q.state = 2;
         m->completed.push_back(q);
         m->h_active[Q.graph] &= ~bit;
@@ -979,6 +1031,55 @@ b40c_status ms_retire(b40c_msbfs* m) {
                            m->rel_buf.as<MsRelease>() + r0, rows);
         B40C_HIP(hipGetLastError());
     }
+    return B40C_OK;
+}
+
+// The five launches of one level (phase argument `ph`: absolute, or the
+// level's index within a captured replay when a.dph is set).
+void ms_launch_level(b40c_msbfs* m, const MsArgs& a, int ph) {
+    hipLaunchKernelGGL(k_ms_push, dim3(m->grid), dim3(BFS_THREADS), 0, m->stream, a, ph);
+    hipLaunchKernelGGL(k_ms_heavy, dim3(m->grid), dim3(BFS_THREADS), 0, m->stream, a, ph);
+    hipLaunchKernelGGL(k_ms_pull, dim3(2 * m->grid), dim3(BFS_THREADS), 0, m->stream, a, ph);
+    hipLaunchKernelGGL(k_ms_count, dim3(m->grid), dim3(BFS_THREADS), 0, m->stream, a, ph);
+    hipLaunchKernelGGL(k_ms_direction, dim3((m->G + BFS_WAVES - 1) / BFS_WAVES), dim3(BFS_THREADS), 0, m->stream, a,
+                       ph);
+}
+
+// Captures levels_per_replay levels plus k_ms_advance into one hipGraph
+// (kept while the launch shape and direction knobs are unchanged; every
+// buffer is fixed for the engine's life).  No memset nodes: counters and
+// masks are cleared by the level kernels themselves.
+b40c_status ms_ensure_loop(b40c_msbfs* m) {
+    const int k = m->levels_per_replay;
+    if (m->loop_exec && m->loop_levels == k && m->loop_grid == m->grid && m->loop_alpha == m->alpha &&
+        m->loop_beta == m->beta)
+        return B40C_OK;
+    m->release_loop();
+    MsArgs a = ms_args(m);
+    a.dph = m->dph.as<int32_t>();
+    B40C_HIP(hipStreamBeginCapture(m->stream, hipStreamCaptureModeThreadLocal));
+    for (int j = 0; j < k; ++j) ms_launch_level(m, a, j);
+    hipLaunchKernelGGL(k_ms_advance, dim3(1), dim3(64), 0, m->stream, a, k);
+    const hipError_t el = hipGetLastError();
+    hipGraph_t graph = nullptr;
+    const hipError_t ec = hipStreamEndCapture(m->stream, &graph);
+    if (el != hipSuccess || ec != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return fail(B40C_ERR_HIP, "batched level loop capture failed: %s",
+                    hipGetErrorString(el != hipSuccess ? el : ec));
+    }
+    hipGraphExec_t exec = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        return fail(B40C_ERR_HIP, "batched level loop: hipGraphInstantiate failed: %s", hipGetErrorString(ei));
+    }
+    m->loop_graph = graph;
+    m->loop_exec = exec;
+    m->loop_levels = k;
+    m->loop_grid = m->grid;
+    m->loop_alpha = m->alpha;
+    m->loop_beta = m->beta;
     return B40C_OK;
 }
 
@@ -1086,15 +1187,15 @@ b40c_status b40c_msbfs_create(b40c_multigraph* mg, int64_t label_budget_bytes, i
     if (e == hipSuccess) e = hipHostMalloc((void**)&m->h_ctr, sizeof(MsCounters), hipHostMallocCoherent);
     if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&m->h_ctr_dev, m->h_ctr, 0);
     if (e == hipSuccess)
-        e = hipHostMalloc((void**)&m->h_newbits, (size_t)std::max(m->G, 1) * 8, hipHostMallocCoherent);
-    if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&m->h_newbits_dev, m->h_newbits, 0);
+        e = hipHostMalloc((void**)&m->h_done, (size_t)max_queries * sizeof(int32_t), hipHostMallocCoherent);
+    if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&m->h_done_dev, m->h_done, 0);
     if (e != hipSuccess) s = fail(B40C_ERR_HIP, "b40c_msbfs_create: %s", hipGetErrorString(e));
     const size_t N = (size_t)m->n;
     if (s == B40C_OK) s = m->visited.alloc(N * 8);
     for (int i = 0; i < 3 && s == B40C_OK; ++i) s = m->front[i].alloc(N * 8);
     if (s == B40C_OK) s = m->active.alloc((size_t)m->G * 8);
     if (s == B40C_OK) s = m->bu.alloc((size_t)m->G * 8);
-    if (s == B40C_OK) s = m->newbits.alloc((size_t)m->G * 8);
+    if (s == B40C_OK) s = m->dph.alloc(sizeof(uint32_t));
     if (s == B40C_OK) s = m->cnt_n.alloc((size_t)m->G * 64 * 4);
     if (s == B40C_OK) s = m->cnt_m.alloc((size_t)m->G * 64 * 8);
     if (s == B40C_OK) s = m->nf_prev.alloc((size_t)m->G * 64 * 4);
@@ -1119,10 +1220,10 @@ b40c_status b40c_msbfs_create(b40c_multigraph* mg, int64_t label_budget_bytes, i
     m->arena.init(m->arena_elems);
     m->h_active.assign(m->G, 0ull);
     std::memset(m->h_ctr, 0, sizeof(MsCounters));
-    std::memset(m->h_newbits, 0, (size_t)m->G * 8);
+    std::memset(m->h_done, 0, (size_t)max_queries * sizeof(int32_t));
     B40C_HIP(hipMemsetAsync(m->visited.ptr, 0, m->visited.bytes, m->stream));
     for (int i = 0; i < 3; ++i) B40C_HIP(hipMemsetAsync(m->front[i].ptr, 0, m->front[i].bytes, m->stream));
-    B40C_HIP(hipMemsetAsync(m->newbits.ptr, 0, m->newbits.bytes, m->stream));
+    B40C_HIP(hipMemsetAsync(m->dph.ptr, 0, m->dph.bytes, m->stream));
     B40C_HIP(hipMemsetAsync(m->bu.ptr, 0, m->bu.bytes, m->stream));
     B40C_HIP(hipMemsetAsync(m->fbm.ptr, 0, m->fbm.bytes, m->stream));
     B40C_HIP(hipMemsetAsync(m->cnt_n.ptr, 0, m->cnt_n.bytes, m->stream));
@@ -1180,28 +1281,47 @@ b40c_status b40c_msbfs_submit(b40c_msbfs* m, int32_t graph, int32_t src, int32_t
     return B40C_OK;
 }
 
-// One level boundary: admit, run one level for every query in flight, retire.
+// One level boundary: admit, run one level (or one captured replay of
+// levels_per_replay levels) for every query in flight, retire.
 b40c_status b40c_msbfs_step(b40c_msbfs* m, int32_t* running, int32_t* waiting) {
     if (!m) return fail(B40C_ERR_INVALID_ARG, "b40c_msbfs_step: null");
     B40C_TRY(ms_admit(m));
     if (!m->running.empty()) {
-        const MsArgs a = ms_args(m);
-        hipLaunchKernelGGL(k_ms_push, dim3(m->grid), dim3(BFS_THREADS), 0, m->stream, a, m->ph);
-        hipLaunchKernelGGL(k_ms_heavy, dim3(m->grid), dim3(BFS_THREADS), 0, m->stream, a, m->ph);
-        hipLaunchKernelGGL(k_ms_pull, dim3(2 * m->grid), dim3(BFS_THREADS), 0, m->stream, a, m->ph);
-        hipLaunchKernelGGL(k_ms_count, dim3(m->grid), dim3(BFS_THREADS), 0, m->stream, a, m->ph);
-        hipLaunchKernelGGL(k_ms_direction, dim3((m->G + BFS_WAVES - 1) / BFS_WAVES), dim3(BFS_THREADS), 0, m->stream,
-                           a, m->ph);
-        B40C_HIP(hipGetLastError());
-        // k_ms_direction has written h_ctr's flags and h_newbits.
+        int levels = 1;
+        if (m->levels_per_replay > 0) {
+            levels = m->levels_per_replay;
+            MsArgs a = ms_args(m);
+            a.dph = m->dph.as<int32_t>();
+            if (m->dev_ph != m->ph) {
+                hipLaunchKernelGGL(k_ms_set_phase, dim3(1), dim3(64), 0, m->stream, a, m->ph);
+                B40C_HIP(hipGetLastError());
+                m->dev_ph = m->ph;
+            }
+            B40C_TRY(ms_ensure_loop(m));
+            B40C_HIP(hipGraphLaunch(m->loop_exec, m->stream));
+            m->dev_ph += levels;
+        } else {
+            const MsArgs a = ms_args(m);
+            ms_launch_level(m, a, m->ph);
+            B40C_HIP(hipGetLastError());
+        }
+        // k_ms_direction has written h_ctr's flags and the finished
+        // queries' done words.
         B40C_HIP(hipStreamSynchronize(m->stream));
         if (m->h_ctr->overflow || m->h_ctr->corrupt)
             return fail(B40C_ERR_HIP, "batched traversal: frontier overflow or corrupt frontier");
         B40C_TRY(ms_retire(m));
-        m->ph += 1;
+        m->ph += levels;
     }
     if (running) *running = (int32_t)m->running.size();
     if (waiting) *waiting = (int32_t)m->waiting.size();
+    return B40C_OK;
+}
+
+b40c_status b40c_msbfs_set_levels_per_step(b40c_msbfs* m, int32_t levels) {
+    if (!m || levels < 0 || levels > 64)
+        return fail(B40C_ERR_INVALID_ARG, "b40c_msbfs_set_levels_per_step: bad arguments");
+    m->levels_per_replay = levels;
     return B40C_OK;
 }
 
@@ -1257,7 +1377,8 @@ void b40c_msbfs_destroy(b40c_msbfs* m) {
     for (int i = 0; i < 3; ++i) m->front[i].release();
     m->active.release();
     m->bu.release();
-    m->newbits.release();
+    m->release_loop();
+    m->dph.release();
     m->cnt_n.release();
     m->cnt_m.release();
     m->nf_prev.release();
@@ -1278,7 +1399,7 @@ void b40c_msbfs_destroy(b40c_msbfs* m) {
     if (m->h_seeds) (void)hipHostFree(m->h_seeds);
     if (m->h_rel) (void)hipHostFree(m->h_rel);
     if (m->h_ctr) (void)hipHostFree(m->h_ctr);
-    if (m->h_newbits) (void)hipHostFree(m->h_newbits);
+    if (m->h_done) (void)hipHostFree(m->h_done);
     if (m->stream) (void)hipStreamDestroy(m->stream);
     b40c_multigraph_destroy(m->mg);
     delete m;

@@ -40,10 +40,8 @@ namespace {
 
 constexpr int RS_THREADS = 256;
 constexpr int RS_WAVES = RS_THREADS / kWave;
-constexpr int RS_ITEMS = 8;
-constexpr int RS_TILE = RS_THREADS * RS_ITEMS;
 constexpr int RS_BINS = 256;
-constexpr int RS_WGS_PER_CU = 4;
+constexpr int RS_MAX_WGS_PER_CU = 8;
 
 // Order-preserving views of the key bits (sort_utils.cuh:87-183):
 // CONV 0 unsigned (as is), 1 signed (flip the sign bit), 2 IEEE float
@@ -77,15 +75,15 @@ __global__ void k_sort_init(uint32_t* digit_total0, int32_t* sel, uint32_t* triv
 // (digit-major) and to the global digit totals.
 template <typename K, int CONV>
 __global__ __launch_bounds__(RS_THREADS) void k_upsweep(const K* k0, const K* k1, const int32_t* sel, int64_t n,
-                                                        int shift, uint32_t mask, int64_t tiles_per_wg,
+                                                        int shift, uint32_t mask, int64_t keys_per_wg,
                                                         uint32_t* spine, uint32_t* digit_total, int G) {
     __shared__ uint32_t hist[RS_WAVES][RS_BINS];
     const int tid = threadIdx.x, wid = tid / kWave;
     for (int i = tid; i < RS_WAVES * RS_BINS; i += RS_THREADS) (&hist[0][0])[i] = 0;
     __syncthreads();
     const K* keys = *sel ? k1 : k0;
-    const int64_t begin = (int64_t)blockIdx.x * tiles_per_wg * RS_TILE;
-    const int64_t end = min(n, begin + tiles_per_wg * RS_TILE);
+    const int64_t begin = (int64_t)blockIdx.x * keys_per_wg;
+    const int64_t end = min(n, begin + keys_per_wg);
     for (int64_t i = begin + tid; i < end; i += RS_THREADS) {
         uint32_t d = digit_of<K, CONV>(keys[i], shift, mask);
         atomicAdd(&hist[wid][d], 1u);
@@ -124,11 +122,16 @@ __global__ __launch_bounds__(RS_THREADS) void k_spine(uint32_t* spine, const uin
     }
 }
 
-template <typename K, int CONV, bool VALS>
+// ITEMS keys per thread: a tile of RS_THREADS * ITEMS keys is ranked and
+// staged in LDS at once.  Larger tiles give longer per-digit runs in the
+// scatter (about ITEMS keys per digit on uniform keys), so fewer partial
+// cache lines are written, for more LDS per workgroup.
+template <typename K, int CONV, bool VALS, int ITEMS>
 __global__ __launch_bounds__(RS_THREADS) void k_downsweep(K* k0, K* k1, uint32_t* v0, uint32_t* v1,
                                                           const int32_t* sel, int64_t n, int shift, uint32_t mask,
-                                                          int64_t tiles_per_wg, const uint32_t* spine, int G,
+                                                          int64_t keys_per_wg, const uint32_t* spine, int G,
                                                           const uint32_t* trivial) {
+    constexpr int RS_TILE = RS_THREADS * ITEMS;
     if (*trivial) return;
     __shared__ K stage_k[RS_TILE];
     __shared__ uint32_t stage_v[VALS ? RS_TILE : 1];
@@ -145,8 +148,8 @@ __global__ __launch_bounds__(RS_THREADS) void k_downsweep(K* k0, K* k1, uint32_t
     uint32_t* vout = VALS ? (s ? v0 : v1) : nullptr;
 
     carry[tid] = spine[(int64_t)tid * G + blockIdx.x];
-    const int64_t begin = (int64_t)blockIdx.x * tiles_per_wg * RS_TILE;
-    const int64_t end = min(n, begin + tiles_per_wg * RS_TILE);
+    const int64_t begin = (int64_t)blockIdx.x * keys_per_wg;
+    const int64_t end = min(n, begin + keys_per_wg);
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 
     for (int64_t tile_base = begin; tile_base < end; tile_base += RS_TILE) {
@@ -154,13 +157,13 @@ __global__ __launch_bounds__(RS_THREADS) void k_downsweep(K* k0, K* k1, uint32_t
         for (int i = lane; i < RS_BINS; i += kWave) wave_hist[wid][i] = 0;
         __syncthreads();
 
-        K key[RS_ITEMS];
-        uint32_t val[RS_ITEMS];
-        uint32_t rank[RS_ITEMS];
-        uint32_t dig[RS_ITEMS];
+        K key[ITEMS];
+        uint32_t val[ITEMS];
+        uint32_t rank[ITEMS];
+        uint32_t dig[ITEMS];
 #pragma unroll
-        for (int j = 0; j < RS_ITEMS; ++j) {
-            const int idx = wid * (kWave * RS_ITEMS) + j * kWave + lane;
+        for (int j = 0; j < ITEMS; ++j) {
+            const int idx = wid * (kWave * ITEMS) + j * kWave + lane;
             const bool valid = idx < tile_n;
             key[j] = valid ? kin[tile_base + idx] : (K)0;
             if (VALS) val[j] = valid ? vin[tile_base + idx] : 0u;
@@ -193,7 +196,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_downsweep(K* k0, K* k1, uint32_t
         tile_excl[tid] = (uint32_t)excl;
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < RS_ITEMS; ++j) {
+        for (int j = 0; j < ITEMS; ++j) {
             if (rank[j] != 0xFFFFFFFFu) {
                 const uint32_t pos = tile_excl[dig[j]] + wave_hist[wid][dig[j]] + rank[j];
                 stage_k[pos] = key[j];
@@ -220,22 +223,26 @@ __global__ void k_flip_selector(int32_t* sel, uint32_t* trivial) {
 }
 
 // Launch shape for n keys on a device with `num_cus` CUs: at most
-// RS_WGS_PER_CU workgroups per CU, each owning a whole number of tiles.
+// policy.wgs_per_cu workgroups per CU, each owning a whole number of tiles.
 struct SortPlan {
-    int64_t tiles_per_wg = 1;
+    int64_t keys_per_wg = 1;
     int G = 1;
+    int items = 8;
 };
 
-SortPlan sort_plan(int64_t n, int num_cus) {
+SortPlan sort_plan(int64_t n, int num_cus, const SortPolicy& pol) {
     SortPlan p;
-    const int64_t num_tiles = std::max<int64_t>(1, (n + RS_TILE - 1) / RS_TILE);
-    const int64_t max_grid = (int64_t)num_cus * RS_WGS_PER_CU;
-    p.tiles_per_wg = (num_tiles + max_grid - 1) / max_grid;
-    p.G = (int)((num_tiles + p.tiles_per_wg - 1) / p.tiles_per_wg);
+    p.items = pol.items;
+    const int64_t tile = (int64_t)RS_THREADS * pol.items;
+    const int64_t num_tiles = std::max<int64_t>(1, (n + tile - 1) / tile);
+    const int64_t max_grid = (int64_t)num_cus * pol.wgs_per_cu;
+    const int64_t tiles_per_wg = (num_tiles + max_grid - 1) / max_grid;
+    p.keys_per_wg = tiles_per_wg * tile;
+    p.G = (int)((num_tiles + tiles_per_wg - 1) / tiles_per_wg);
     return p;
 }
 
-template <typename K, int CONV, bool VALS>
+template <typename K, int CONV, bool VALS, int ITEMS>
 void launch_passes(K* k0, K* k1, uint32_t* v0, uint32_t* v1, int64_t n, int begin_bit, int end_bit,
                    const SortPlan& plan, const SortScratch& sc, hipStream_t stream) {
     uint32_t* dtot[2] = {sc.digit_total, sc.digit_total + RS_BINS};
@@ -247,11 +254,11 @@ void launch_passes(K* k0, K* k1, uint32_t* v0, uint32_t* v1, int64_t n, int begi
         uint32_t* cur = dtot[pass & 1];
         uint32_t* next = dtot[(pass + 1) & 1];
         hipLaunchKernelGGL((k_upsweep<K, CONV>), dim3(plan.G), dim3(RS_THREADS), 0, stream, k0, k1, sc.sel, n, shift,
-                           mask, plan.tiles_per_wg, sc.spine, cur, plan.G);
+                           mask, plan.keys_per_wg, sc.spine, cur, plan.G);
         hipLaunchKernelGGL(k_spine, dim3(RS_BINS), dim3(RS_THREADS), 0, stream, sc.spine, cur, next, plan.G, n,
                            sc.trivial);
-        hipLaunchKernelGGL((k_downsweep<K, CONV, VALS>), dim3(plan.G), dim3(RS_THREADS), 0, stream, k0, k1, v0, v1,
-                           sc.sel, n, shift, mask, plan.tiles_per_wg, sc.spine, plan.G, sc.trivial);
+        hipLaunchKernelGGL((k_downsweep<K, CONV, VALS, ITEMS>), dim3(plan.G), dim3(RS_THREADS), 0, stream, k0, k1, v0,
+                           v1, sc.sel, n, shift, mask, plan.keys_per_wg, sc.spine, plan.G, sc.trivial);
         hipLaunchKernelGGL(k_flip_selector, dim3(1), dim3(1), 0, stream, sc.sel, sc.trivial);
     }
 }
@@ -259,10 +266,19 @@ void launch_passes(K* k0, K* k1, uint32_t* v0, uint32_t* v1, int64_t n, int begi
 template <typename K, int CONV>
 void launch_typed(void* k0, void* k1, uint32_t* v0, uint32_t* v1, int64_t n, int begin_bit, int end_bit,
                   const SortPlan& plan, const SortScratch& sc, hipStream_t stream) {
-    if (v0)
-        launch_passes<K, CONV, true>((K*)k0, (K*)k1, v0, v1, n, begin_bit, end_bit, plan, sc, stream);
-    else
-        launch_passes<K, CONV, false>((K*)k0, (K*)k1, nullptr, nullptr, n, begin_bit, end_bit, plan, sc, stream);
+    if (v0) {
+        if (plan.items == 16)
+            launch_passes<K, CONV, true, 16>((K*)k0, (K*)k1, v0, v1, n, begin_bit, end_bit, plan, sc, stream);
+        else
+            launch_passes<K, CONV, true, 8>((K*)k0, (K*)k1, v0, v1, n, begin_bit, end_bit, plan, sc, stream);
+    } else {
+        if (plan.items == 16)
+            launch_passes<K, CONV, false, 16>((K*)k0, (K*)k1, nullptr, nullptr, n, begin_bit, end_bit, plan, sc,
+                                              stream);
+        else
+            launch_passes<K, CONV, false, 8>((K*)k0, (K*)k1, nullptr, nullptr, n, begin_bit, end_bit, plan, sc,
+                                             stream);
+    }
 }
 
 int key_bits(int32_t key_type) {
@@ -294,8 +310,17 @@ b40c_status validate(const char* who, int32_t key_type, const void* k0, const vo
 
 }  // namespace
 
-size_t sort_scratch_words(int64_t n, int num_cus) {
-    const SortPlan p = sort_plan(n, num_cus);
+// Defaults: 8 keys per thread and 4 workgroups per CU.
+SortPolicy resolve_sort_policy(SortPolicy p, int64_t n, int key_bytes) {
+    (void)n;
+    (void)key_bytes;
+    if (p.items != 8 && p.items != 16) p.items = 8;
+    if (p.wgs_per_cu < 1 || p.wgs_per_cu > RS_MAX_WGS_PER_CU) p.wgs_per_cu = 4;
+    return p;
+}
+
+size_t sort_scratch_words(int64_t n, int num_cus, const SortPolicy& pol) {
+    const SortPlan p = sort_plan(n, num_cus, pol);
     return (size_t)RS_BINS * (size_t)p.G + 2 * RS_BINS + 2;
 }
 
@@ -310,14 +335,15 @@ SortScratch sort_scratch_view(uint32_t* words, size_t count) {
 }
 
 b40c_status radix_sort_async(int32_t key_type, void* k0, void* k1, uint32_t* v0, uint32_t* v1, int64_t n,
-                             int begin_bit, int end_bit, int num_cus, const SortScratch& sc, hipStream_t stream,
-                             bool& launched) {
+                             int begin_bit, int end_bit, int num_cus, const SortPolicy& policy,
+                             const SortScratch& sc, hipStream_t stream, bool& launched) {
     launched = false;
     if (n <= 1 || begin_bit == end_bit) return B40C_OK;
-    const SortPlan plan = sort_plan(n, num_cus);
-    if (sort_scratch_words(n, num_cus) > sc.words)
+    const SortPolicy pol = resolve_sort_policy(policy, n, key_bits(key_type) / 8);
+    const SortPlan plan = sort_plan(n, num_cus, pol);
+    if (sort_scratch_words(n, num_cus, pol) > sc.words)
         return fail(B40C_ERR_INVALID_ARG, "radix sort: scratch of %zu words is smaller than the %zu needed", sc.words,
-                    sort_scratch_words(n, num_cus));
+                    sort_scratch_words(n, num_cus, pol));
     switch (key_type) {
         case B40C_KEY_U32: launch_typed<uint32_t, KC_UNSIGNED>(k0, k1, v0, v1, n, begin_bit, end_bit, plan, sc, stream); break;
         case B40C_KEY_I32: launch_typed<uint32_t, KC_SIGNED>(k0, k1, v0, v1, n, begin_bit, end_bit, plan, sc, stream); break;
@@ -335,9 +361,11 @@ b40c_status radix_sort_async(int32_t key_type, void* k0, void* k1, uint32_t* v0,
 // Runs a sort with its scratch and reads the selector back (one stream
 // synchronization).
 b40c_status radix_sort_sync(int32_t key_type, void* k0, void* k1, uint32_t* v0, uint32_t* v1, int64_t n, int begin_bit,
-                            int end_bit, int num_cus, const SortScratch& sc, int32_t* selector, hipStream_t stream) {
+                            int end_bit, int num_cus, const SortPolicy& policy, const SortScratch& sc,
+                            int32_t* selector, hipStream_t stream) {
     bool launched = false;
-    B40C_TRY(radix_sort_async(key_type, k0, k1, v0, v1, n, begin_bit, end_bit, num_cus, sc, stream, launched));
+    B40C_TRY(radix_sort_async(key_type, k0, k1, v0, v1, n, begin_bit, end_bit, num_cus, policy, sc, stream,
+                              launched));
     int32_t h_sel = 0;
     if (launched) {
         B40C_HIP(hipMemcpyAsync(&h_sel, sc.sel, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
@@ -355,9 +383,10 @@ b40c_status radix_sort_once(int32_t key_type, void* k0, void* k1, uint32_t* v0, 
     DeviceInfo info;
     B40C_TRY(device_info(info));
     DeviceBuffer mem;
-    const size_t words = sort_scratch_words(n, info.num_cus);
+    const SortPolicy pol = resolve_sort_policy(SortPolicy{}, n, key_bits(key_type) / 8);
+    const size_t words = sort_scratch_words(n, info.num_cus, pol);
     B40C_TRY(mem.alloc(words * sizeof(uint32_t)));
-    return radix_sort_sync(key_type, k0, k1, v0, v1, n, begin_bit, end_bit, info.num_cus,
+    return radix_sort_sync(key_type, k0, k1, v0, v1, n, begin_bit, end_bit, info.num_cus, pol,
                            sort_scratch_view(mem.as<uint32_t>(), words), selector, stream);
 }
 
@@ -378,16 +407,16 @@ b40c_status radix_sort_pairs_u32(uint32_t* k0, uint32_t* k1, uint32_t* v0, uint3
 // ---------------------------------------------------------------------------
 // Allocation-free sort (no counterpart in the reference): its scratch is this
 // fixed device array, one copy per GPU, which the GPU keeps until the
-// process ends.  The grid is capped at RS_WGS_PER_CU workgroups per CU, so
-// kNoAllocWords covers a sort of any n <= 2^32 - 1 on a device with up to
-// kNoAllocMaxCus CUs (256 on MI355X): 1 MiB + 2 KiB.  In host code `scratch`
+// process ends.  The grid is capped at RS_MAX_WGS_PER_CU workgroups per CU,
+// so kNoAllocWords covers a sort of any n <= 2^32 - 1 under any policy on a
+// device with up to kNoAllocMaxCus CUs (256 on MI355X): 2 MiB + 2 KiB.  In host code `scratch`
 // is only a stand-in for the device array; its device address comes from
 // hipGetSymbolAddress, once per device, and only that address is used.
 // ---------------------------------------------------------------------------
 namespace b40c {
 namespace {
 constexpr int kNoAllocMaxCus = 256;
-constexpr size_t kNoAllocWords = (size_t)RS_BINS * RS_WGS_PER_CU * kNoAllocMaxCus + 2 * RS_BINS + 64;
+constexpr size_t kNoAllocWords = (size_t)RS_BINS * RS_MAX_WGS_PER_CU * kNoAllocMaxCus + 2 * RS_BINS + 64;
 }  // namespace
 }  // namespace b40c
 
@@ -431,6 +460,7 @@ b40c_status noalloc_slot(NoAllocSlot*& out) {
 struct b40c_sort_enactor {
     b40c::DeviceBuffer scratch;
     hipStream_t stream = nullptr;
+    b40c::SortPolicy policy;  // zero fields: the size-based default
 };
 
 using namespace b40c;
@@ -467,13 +497,14 @@ b40c_status b40c_radix_sort_u64_noalloc(uint64_t* d_keys0, uint64_t* d_keys1, in
     if (n <= 1 || begin_bit == end_bit) return B40C_OK;
     NoAllocSlot* slot = nullptr;
     B40C_TRY(noalloc_slot(slot));
-    const size_t need = sort_scratch_words(n, slot->num_cus);
+    const SortPolicy pol = resolve_sort_policy(SortPolicy{}, n, 8);
+    const size_t need = sort_scratch_words(n, slot->num_cus, pol);
     if (need > kNoAllocWords)
         return fail(B40C_ERR_UNSUPPORTED,
                     "b40c_radix_sort_u64_noalloc: n=%lld on %d CUs needs %zu scratch words; the fixed array holds %zu",
                     (long long)n, slot->num_cus, need, kNoAllocWords);
     return radix_sort_sync(B40C_KEY_U64, d_keys0, d_keys1, nullptr, nullptr, n, begin_bit, end_bit, slot->num_cus,
-                           sort_scratch_view(slot->ptr, kNoAllocWords), selector, nullptr);
+                           pol, sort_scratch_view(slot->ptr, kNoAllocWords), selector, nullptr);
 }
 
 b40c_status b40c_sort_enactor_create(b40c_sort_enactor** out) {
@@ -498,12 +529,25 @@ b40c_status b40c_sort_enactor_sort(b40c_sort_enactor* e, int32_t key_type, void*
     if (n <= 1 || begin_bit == end_bit) return B40C_OK;
     DeviceInfo info;
     B40C_TRY(device_info(info));
-    const size_t words = sort_scratch_words(n, info.num_cus);
+    const SortPolicy pol = resolve_sort_policy(e->policy, n, key_bits(key_type) / 8);
+    const size_t words = sort_scratch_words(n, info.num_cus, pol);
     // Grow-only: later sorts of at most this size allocate nothing.
     B40C_TRY(e->scratch.reserve(words * sizeof(uint32_t)));
-    return radix_sort_sync(key_type, d_keys0, d_keys1, d_vals0, d_vals1, n, begin_bit, end_bit, info.num_cus,
+    return radix_sort_sync(key_type, d_keys0, d_keys1, d_vals0, d_vals1, n, begin_bit, end_bit, info.num_cus, pol,
                            sort_scratch_view(e->scratch.as<uint32_t>(), e->scratch.bytes / sizeof(uint32_t)), selector,
                            e->stream);
+}
+
+b40c_status b40c_sort_enactor_set_policy(b40c_sort_enactor* e, int32_t items_per_thread, int32_t wgs_per_cu) {
+    if (!e) return fail(B40C_ERR_INVALID_ARG, "b40c_sort_enactor_set_policy: null enactor");
+    if ((items_per_thread != 0 && items_per_thread != 8 && items_per_thread != 16) || wgs_per_cu < 0 ||
+        wgs_per_cu > RS_MAX_WGS_PER_CU)
+        return fail(B40C_ERR_INVALID_ARG,
+                    "b40c_sort_enactor_set_policy: items_per_thread must be 0, 8 or 16 and wgs_per_cu 0-%d (got %d, %d)",
+                    RS_MAX_WGS_PER_CU, items_per_thread, wgs_per_cu);
+    e->policy.items = items_per_thread;
+    e->policy.wgs_per_cu = wgs_per_cu;
+    return B40C_OK;
 }
 
 void b40c_sort_enactor_destroy(b40c_sort_enactor* e) {

@@ -68,6 +68,11 @@ class Enactor:
         check(lib().b40c_sort_enactor_create(ctypes.byref(h)))
         self._h = h
 
+    def set_policy(self, items_per_thread=0, wgs_per_cu=0):
+        """Launch policy (0 = the tuned default): keys per thread of a
+        downsweep tile (8 or 16) and workgroups per CU (1-8)."""
+        check(lib().b40c_sort_enactor_set_policy(self._h, int(items_per_thread), int(wgs_per_cu)))
+
     def Sort(self, storage, num_elements, begin_bit=0, end_bit=None):
         sel = ctypes.c_int32()
         kt = KEY_TYPES.get(storage.key_dtype)

@@ -62,10 +62,12 @@ def poisson_trace(rate_qps, duration_s, choose, seed=9):
 
 
 class ContinuousServer:
-    def __init__(self, multigraph, label_budget_bytes, max_queries=1 << 20, tuning=None, log=None, stall_s=10.0):
+    def __init__(self, multigraph, label_budget_bytes, max_queries=1 << 20, tuning=None, log=None, stall_s=10.0,
+                 levels_per_step=0):
         self.eng = MultiSourceBFS(multigraph, label_budget_bytes, max_queries)
         if tuning:
             self.eng.set_tuning(**tuning)
+        self.eng.set_levels_per_step(levels_per_step)
         self.mg = multigraph
         self.log = log
         self.stall_s = stall_s

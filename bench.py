@@ -160,6 +160,9 @@ def bench_lattice(bfs, b4, hip, info, width=200, sources=4, steps=8):
     return out
 
 
+BATCH_LEVELS_PER_STEP = 4
+
+
 def bench_batched(b4, info):
     """1,024 queries over 256 RMAT graphs (scale integer-uniform in [10,18],
     4 sources each): the batched multi-source engine vs the plain
@@ -185,46 +188,63 @@ def bench_batched(b4, info):
         plain_ms.append((time.perf_counter() - t1) * 1e3)
     plain_s = time.perf_counter() - t0
     budget = int(sum(graphs[g].nodes for g, _ in queries) * 8 * 1.05) + (1 << 20)
-    eng = MultiSourceBFS(mg, budget, len(queries) + 8)
-    ids = {}
-    done_ms = {}
-    t0 = time.perf_counter()
-    for g, s in queries:
-        ids[eng.submit(g, s)] = (g, s)
-    levels = 0
-    while True:
-        r, w = eng.step()
-        levels += 1
-        now = (time.perf_counter() - t0) * 1e3
-        for q in eng.poll():
-            done_ms[q] = now
-        if r == 0 and w == 0:
-            break
-    batch_s = time.perf_counter() - t0
-    # labels: batched vs plain, every query
-    mismatches = 0
-    for q, (g, s) in ids.items():
-        d_b, _, _ = eng.result(q, g, want_pred=False)
-        d_p, _ = plain.run_one(g, s, want=True)
-        mismatches += int(not np.array_equal(d_b, d_p))
-    lat = list(done_ms.values())
+
+    def run_batched(levels_per_step):
+        eng = MultiSourceBFS(mg, budget, len(queries) + 8)
+        eng.set_levels_per_step(levels_per_step)
+        ids, done_ms = {}, {}
+        t0 = time.perf_counter()
+        for g, s in queries:
+            ids[eng.submit(g, s)] = (g, s)
+        steps = 0
+        while True:
+            r, w = eng.step()
+            steps += 1
+            now = (time.perf_counter() - t0) * 1e3
+            for q in eng.poll():
+                done_ms[q] = now
+            if r == 0 and w == 0:
+                break
+        return eng, ids, done_ms, steps, time.perf_counter() - t0
+
+    # Two launch forms of the same engine: five launches and a host
+    # synchronisation per level, and a captured hipGraph replaying
+    # BATCH_LEVELS_PER_STEP levels per synchronisation.
+    forms = {}
+    for name, lps in (("launch", 0), ("graph", BATCH_LEVELS_PER_STEP)):
+        run_batched(lps)[0].close()  # warm-up (capture, first-touch of the label arena)
+        eng, ids, done_ms, steps, batch_s = run_batched(lps)
+        mismatches = 0
+        for q, (g, s) in ids.items():
+            d_b, _, _ = eng.result(q, g, want_pred=False)
+            d_p, _ = plain.run_one(g, s, want=True)
+            mismatches += int(not np.array_equal(d_b, d_p))
+        eng.close()
+        lat = list(done_ms.values())
+        forms[name] = {"levels_per_step": lps, "steps": steps, "batched_qps": round(len(queries) / batch_s, 1),
+                       "speedup": round(plain_s / batch_s, 2), "batched_ms_p50": round(pct(lat, 50), 3),
+                       "batched_ms_p99": round(pct(lat, 99), 3), "label_mismatches": mismatches}
+        log("batched %s: %s" % (name, forms[name]))
+    best = max(forms.values(), key=lambda f: f["batched_qps"])
     # The same arrival for the plain enactor: all queries submitted at once
     # and served one after another, so a query finishes when every query
     # before it and itself have run.
     plain_done = np.cumsum(plain_ms).tolist()
+    mismatches = sum(f["label_mismatches"] for f in forms.values())
     res = {
-        "queries": len(queries), "graphs": len(graphs), "levels": levels,
-        "plain_qps": round(len(queries) / plain_s, 1), "batched_qps": round(len(queries) / batch_s, 1),
-        "speedup": round(plain_s / batch_s, 2),
+        "queries": len(queries), "graphs": len(graphs), "levels_per_step": best["levels_per_step"],
+        "plain_qps": round(len(queries) / plain_s, 1), "batched_qps": best["batched_qps"],
+        "speedup": best["speedup"],
         "plain_ms_p50": round(pct(plain_ms, 50), 4), "plain_ms_p99": round(pct(plain_ms, 99), 4),
-        "batched_ms_p50": round(pct(lat, 50), 3), "batched_ms_p99": round(pct(lat, 99), 3),
+        "batched_ms_p50": best["batched_ms_p50"], "batched_ms_p99": best["batched_ms_p99"],
         "plain_fifo_ms_p50": round(pct(plain_done, 50), 3), "plain_fifo_ms_p99": round(pct(plain_done, 99), 3),
+        "forms": forms,
         "labels_identical": mismatches == 0, "label_mismatches": mismatches,
         "note": "batched latency = batch start -> query finished (all 1,024 submitted at once); plain_ms = one "
                 "query alone (Reset + EnactSearch, direction-optimizing); plain_fifo_ms = the same all-at-once "
-                "arrival served one query after another by the plain enactor",
+                "arrival served one query after another by the plain enactor; headline fields from the faster "
+                "launch form, labels of both forms compared with the plain enactor's for every query",
     }
-    eng.close()
     plain.close()
     mg.close()
     log("batched:", res)
@@ -276,19 +296,24 @@ def bench_served(b4, info, duration_s=1.5):
     out["points"].append({"server": "plain", "offered_qps": round(rate_plain, 1), "queries": len(trace),
                           "achieved_qps": round(len(trace) / wall, 1), "p50_ms": round(pct(lat, 50) * 1e3, 3),
                           "p99_ms": round(pct(lat, 99) * 1e3, 3)})
-    cont = serve.ContinuousServer(mg, label_budget_bytes=6 << 30, log=log)
-    for mult in (2, 3, 4, 5, 10):
-        rate = mult * rate_plain
-        trace = serve.poisson_trace(rate, duration_s, choose, seed=21 + mult)
-        log("served: continuous trace of %d queries at %.0f qps" % (len(trace), rate))
-        lat, wall = cont.serve(trace)
-        achieved = len(trace) / wall
-        out["points"].append({"server": "continuous", "offered_qps": round(rate, 1), "queries": len(trace),
-                              "achieved_qps": round(achieved, 1), "p50_ms": round(pct(lat, 50) * 1e3, 3),
-                              "p99_ms": round(pct(lat, 99) * 1e3, 3)})
-        log("served: point", out["points"][-1])
-        if achieved < 0.8 * rate:
-            break  # saturated: higher offered rates only lengthen the queue
+    # Continuous batching in both launch forms: one level per step with five
+    # launches, and one captured single-level hipGraph per step.
+    for lps in (0, 1):
+        cont = serve.ContinuousServer(mg, label_budget_bytes=6 << 30, log=log, levels_per_step=lps)
+        for mult in (2, 3, 4, 5, 10):
+            rate = mult * rate_plain
+            trace = serve.poisson_trace(rate, duration_s, choose, seed=21 + mult)
+            log("served: continuous (levels_per_step %d) trace of %d queries at %.0f qps" % (lps, len(trace), rate))
+            lat, wall = cont.serve(trace)
+            achieved = len(trace) / wall
+            out["points"].append({"server": "continuous", "levels_per_step": lps, "offered_qps": round(rate, 1),
+                                  "queries": len(trace), "achieved_qps": round(achieved, 1),
+                                  "p50_ms": round(pct(lat, 50) * 1e3, 3), "p99_ms": round(pct(lat, 99) * 1e3, 3)})
+            log("served: point", out["points"][-1])
+            if achieved < 0.8 * rate:
+                break  # saturated: higher offered rates only lengthen the queue
+        if lps == 0:
+            cont.eng.close()
     # Alone vs in the mix: labels of a sample of queries served both ways.
     sample = [choose(serve._uniform(99, k), k) for k in range(24)]
     ids = {cont.eng.submit(g, s): (g, s) for g, s in sample}
@@ -701,7 +726,8 @@ def main():
     ap.add_argument("--no-lattice", action="store_true", help="skip the 200^3 lattice (high-diameter) workload")
     ap.add_argument("--no-ingest", action="store_true",
                     help="skip the graph-file read/build, primitives and per-stage microbenchmark workloads")
-    ap.add_argument("--only", choices=["batched", "served"], help=argparse.SUPPRESS)
+    ap.add_argument("--only", choices=["batched", "served", "lattice", "sort"],
+                    help="run one secondary workload alone and print its result (profiling runs)")
     ap.add_argument("--probe", action="store_true",
                     help="counter-collection run: a 1 GiB copy plus --steps searches, no JSON line")
     args = ap.parse_args()
@@ -718,7 +744,14 @@ def main():
         import back40computing_amd as b4
         from back40computing_amd import bfs
         info = bfs.device_info()
-        res = bench_batched(b4, info) if args.only == "batched" else bench_served(b4, info)
+        if args.only == "batched":
+            res = bench_batched(b4, info)
+        elif args.only == "served":
+            res = bench_served(b4, info)
+        elif args.only == "lattice":
+            res = bench_lattice(bfs, b4, Hip(), info)
+        else:
+            res = bench_sort(Hip())
         print("RESULT " + json.dumps(res), flush=True)
         return
 

@@ -78,6 +78,15 @@ public:
         return sort(storage, (int64_t)num_elements, 0, (int)(8 * sizeof(K)));
     }
 
+    // Launch policy chosen at run time (the reference picks a KernelPolicy
+    // per architecture at compile time, radix_sort/autotuned_policy.cuh:
+    // 190-230): keys per thread of a tile (8 or 16) and workgroups per CU
+    // (1-8); 0 keeps the tuned default.
+    b40c_status SetPolicy(int items_per_thread, int wgs_per_cu) {
+        if (!e_) return B40C_ERR_HIP;
+        return b40c_sort_enactor_set_policy(e_, items_per_thread, wgs_per_cu);
+    }
+
 private:
     template <typename K, typename V>
     b40c_status sort(util::DoubleBuffer<K, V>& s, int64_t n, int begin_bit, int end_bit) {

@@ -216,6 +216,12 @@ b40c_status b40c_sort_enactor_create(b40c_sort_enactor** out);
 b40c_status b40c_sort_enactor_sort(b40c_sort_enactor* e, int32_t key_type, void* d_keys0, void* d_keys1,
                                    uint32_t* d_vals0, uint32_t* d_vals1, int64_t n, int32_t begin_bit,
                                    int32_t end_bit, int32_t* selector);
+/* The enactor's launch policy, the run-time form of the reference's
+ * per-architecture sort policies (radix_sort/autotuned_policy.cuh:190-230):
+ * keys per thread of a downsweep tile (8 or 16; 0 = default) and
+ * workgroups per CU (1-8; 0 = default).  The defaults are the tuned values
+ * the stateless sorts and the CSR builds use. */
+b40c_status b40c_sort_enactor_set_policy(b40c_sort_enactor* e, int32_t items_per_thread, int32_t wgs_per_cu);
 void b40c_sort_enactor_destroy(b40c_sort_enactor* e);
 /* Device allocation helpers so FFI callers can stage radix-sort buffers. */
 b40c_status b40c_device_alloc(void** d_ptr, int64_t bytes);
@@ -293,6 +299,13 @@ b40c_status b40c_msbfs_set_tuning(b40c_msbfs* m, int32_t alpha, int32_t beta, in
  * queries arrive in cohorts that share their expensive first levels (1 =
  * admit at every boundary; default 6).  An idle engine admits at once. */
 b40c_status b40c_msbfs_set_admission(b40c_msbfs* m, int32_t cohort_gap);
+/* Launch form of a step (north-star batched form; no counterpart in the
+ * reference).  levels >= 1: each b40c_msbfs_step replays one hipGraph that
+ * holds `levels` levels (five kernels each, the phase read from device
+ * memory) and one phase-advance node, then synchronizes once; queries that
+ * finish inside the replay are retired on the device and reported at its
+ * end.  0 (default): each step launches one level's kernels directly. */
+b40c_status b40c_msbfs_set_levels_per_step(b40c_msbfs* m, int32_t levels);
 /* Queues a query (source `src` local to graph `graph`); returns its id.
  * B40C_ERR_ADMISSION if its labels can never fit the budget. */
 b40c_status b40c_msbfs_submit(b40c_msbfs* m, int32_t graph, int32_t src, int32_t* query_id);

@@ -160,6 +160,15 @@ def check_pred(offsets, cols, ref_depth, pred):
                                     np.ascontiguousarray(pred, np.int32)))
 
 
+def check_pred_many(offsets, cols, ref_depths, preds, threads=16):
+    """check_pred for several searches of one graph, `threads` at a time
+    (ctypes releases the GIL around each call).  Returns the list of results
+    in order."""
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=max(1, threads)) as ex:
+        return list(ex.map(lambda dp: check_pred(offsets, cols, dp[0], dp[1]), zip(ref_depths, preds)))
+
+
 def edges_visited(offsets, labels):
     nv = ctypes.c_int64()
     e = lib().orc_edges_visited(offsets.size - 1, offsets, np.ascontiguousarray(labels, np.int32),

@@ -100,7 +100,7 @@ K random_key(uint64_t& seed) {
 }
 
 template <typename K>
-bool typed_case(const char* name, size_t n, uint64_t seed, bool with_values) {
+bool typed_case(const char* name, size_t n, uint64_t seed, bool with_values, int items = 0, int wgs = 0) {
     std::vector<K> k(n);
     std::vector<uint32_t> v(n);
     for (size_t i = 0; i < n; ++i) {
@@ -112,10 +112,11 @@ bool typed_case(const char* name, size_t n, uint64_t seed, bool with_values) {
     uint32_t* dv0 = device_copy(v);
     uint32_t* dv1 = device_copy(std::vector<uint32_t>(n));
     b40c::radix_sort::Enactor sorter;
-    b40c_status s;
+    b40c_status s = sorter.SetPolicy(items, wgs);
     std::vector<K> gk;
     std::vector<uint32_t> gv;
-    if (with_values) {
+    if (s != B40C_OK) {
+    } else if (with_values) {
         b40c::util::DoubleBuffer<K, uint32_t> buf(dk0, dk1, dv0, dv1);
         s = sorter.Sort(buf, n);
         gk = host_copy(buf.d_keys[buf.selector], n);
@@ -131,7 +132,8 @@ bool typed_case(const char* name, size_t n, uint64_t seed, bool with_values) {
     std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return k[a] < k[b]; });
     bool ok = s == B40C_OK;
     for (size_t i = 0; ok && i < n; ++i) ok = gk[i] == k[order[i]] && (!with_values || gv[i] == v[order[i]]);
-    printf("%s%s n=%zu: %s\n", name, with_values ? " pairs" : " keys", n, ok ? "CORRECT" : "INCORRECT");
+    printf("%s%s n=%zu policy=%d/%d: %s\n", name, with_values ? " pairs" : " keys", n, items, wgs,
+           ok ? "CORRECT" : "INCORRECT");
     return ok;
 }
 
@@ -232,6 +234,8 @@ int main(int argc, char** argv) {
     ok &= typed_case<float>("float", n, seed + 11, false);
     ok &= typed_case<double>("double", n, seed + 12, true);
     ok &= typed_case<uint32_t>("uint32", n, seed + 13, false);
+    ok &= typed_case<uint64_t>("uint64", n, seed + 14, true, 16, 2);  // a non-default launch policy
+    ok &= typed_case<int32_t>("int32", n, seed + 15, false, 16, 8);
     ok &= float_specials_case();
     ok &= scan_case(n, seed + 7);
     ok &= scan_case(0, seed + 8);

@@ -271,13 +271,16 @@ def test_rmat22_headline_sources(bfsmod, orc, rmat22, strategy, mode, nsrc):
     g, off, cols, srcs, refs = rmat22
     info = bfsmod.device_info()
     tuning = autotune.lookup(info["arch"], strategy, g.nodes, g.edges)
+    preds = []
     for k, s in enumerate(srcs[:nsrc]):
         depth, pred, st = run(bfsmod, g, s, strategy, mode, tuning=tuning)
         ref = refs[k]
         sd = int(ref.max()) + 1
         np.testing.assert_array_equal(depth, ref)
-        assert orc.check_pred(off, cols, ref, pred) == -1
         assert st["search_depth"] == sd
+        preds.append(pred)
+    # Predecessor checks on the host, several sources at a time.
+    assert orc.check_pred_many(off, cols, refs[:nsrc], preds) == [-1] * nsrc
 
 
 @pytest.fixture(scope="module")
@@ -307,9 +310,12 @@ def test_lattice200_closed_form(bfsmod, orc, lattice200, strategy, mode, nsrc):
     w = 200
     fixed = [int(perm[0]), int(perm[w ** 3 - 1]), int(perm[w ** 3 // 2 + 100])]
     seeded = [int(x) for x in orc.pick_sources(off, 3, 13)]
+    refs, preds = [], []
     for s in (fixed + seeded)[:nsrc]:
         depth, pred, st = run(bfsmod, g, s, strategy, mode)
         ref = orc.lattice_depth(200, perm, s)
         np.testing.assert_array_equal(depth, ref)
-        assert orc.check_pred(off, cols, ref, pred) == -1
         assert st["search_depth"] == int(ref.max()) + 1
+        refs.append(ref)
+        preds.append(pred)
+    assert orc.check_pred_many(off, cols, refs, preds) == [-1] * len(refs)

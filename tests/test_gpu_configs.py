@@ -51,12 +51,14 @@ def test_batched_1024_queries_against_oracle(mods, orc):
     for g, srcs in by_graph.items():
         off, cols = graphs[g].to_host()
         refs = orc.bfs_many(off, cols, srcs, threads=16)
+        preds = []
         for k, s in enumerate(srcs):
             depth, pred, sd = results[(g, s)]
             np.testing.assert_array_equal(depth, refs[k], err_msg="graph %d source %d" % (g, s))
-            assert orc.check_pred(off, cols, refs[k], pred) == -1, "graph %d source %d" % (g, s)
             assert sd == int(refs[k].max()) + 1
+            preds.append(pred)
             checked += 1
+        assert orc.check_pred_many(off, cols, refs, preds) == [-1] * len(srcs), "graph %d" % g
     assert checked == 1024
     for g in graphs:
         g.close()
@@ -104,17 +106,22 @@ def test_served_rmat23_pool_against_oracle(mods, orc):
     plain = serve.PlainServer(graphs, bfs._lib.DIRECTION_OPTIMIZING)
     big_srcs = [s for g, s in sample if g == 0]
     big_refs = orc.bfs_many(big_off, big_cols, big_srcs, threads=16) if big_srcs else []
+    big_checks = []
     for (g, s), (depth, pred, sd) in mix.items():
         if g == 0:
             off, cols, ref = big_off, big_cols, big_refs[big_srcs.index(s)]
+            big_checks.append((ref, pred))
         else:
             off, cols = pool[g - 1].to_host()
             ref = orc.bfs(off, cols, s, with_pred=False)[0]
+            assert orc.check_pred(off, cols, ref, pred) == -1
         np.testing.assert_array_equal(depth, ref, err_msg="graph %d source %d" % (g, s))
-        assert orc.check_pred(off, cols, ref, pred) == -1
         assert sd == int(ref.max()) + 1
         alone, _ = plain.run_one(g, s, want=True)
         np.testing.assert_array_equal(alone, depth)
+    # The RMAT-23 queries' predecessors, several at a time on the host.
+    assert orc.check_pred_many(big_off, big_cols, [r for r, _ in big_checks],
+                               [p for _, p in big_checks]) == [-1] * len(big_checks)
     plain.close()
     cont.eng.close()
     mg.close()

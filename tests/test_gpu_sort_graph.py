@@ -110,6 +110,58 @@ def test_sort_enactor_reuses_scratch(rs, orc):
         np.testing.assert_array_equal(rs.sort_u64(k), orc.sort_u64(k))
 
 
+@pytest.mark.parametrize("items,wgs", [(8, 1), (8, 4), (8, 8), (16, 1), (16, 3), (16, 4), (16, 8)])
+def test_sort_policies(rs, orc, items, wgs):
+    """Every launch policy (keys per thread 8/16, 1-8 workgroups per CU)
+    sorts bit-exactly: 64-bit keys alone, uint32 pairs (stable) and signed
+    64-bit keys with values, at sizes around tile and grid boundaries."""
+    from back40computing_amd import _lib
+    en = rs.Enactor()
+    en.set_policy(items, wgs)
+    rng = np.random.default_rng(items * 16 + wgs)
+    for n in (1, 4095, 4097, 256 * 16 * 256 * wgs + 17, 1 << 21):
+        print("sort policy items=%d wgs=%d n=%d" % (items, wgs, n), flush=True)
+        k = rng.integers(0, 2**64 - 1, n, dtype=np.uint64)
+        db = rs.DoubleBuffer(np.uint64, n)
+        db.d_keys[0].upload(k)
+        assert en.Sort(db, n) == 0
+        np.testing.assert_array_equal(db.d_keys[db.selector].download(np.uint64, n), orc.sort_u64(k))
+        db.d_keys[0].free()
+        db.d_keys[1].free()
+        k32 = rng.integers(0, 1000, n).astype(np.uint32)
+        v = np.arange(n, dtype=np.uint32)
+        order = np.argsort(k32, kind="stable")
+        db = rs.DoubleBuffer(np.uint32, n, with_values=True)
+        db.d_keys[0].upload(k32)
+        db.d_values[0].upload(v)
+        assert en.Sort(db, n) == 0
+        np.testing.assert_array_equal(db.d_keys[db.selector].download(np.uint32, n), k32[order])
+        np.testing.assert_array_equal(db.d_values[db.selector].download(np.uint32, n), order.astype(np.uint32))
+        for a in db.d_keys + db.d_values:
+            a.free()
+        ki = rng.integers(-2**63, 2**63 - 1, min(n, 300001), dtype=np.int64)
+        sk, sv = rs.sort(ki, np.arange(ki.size, dtype=np.uint32))
+        order = np.argsort(ki, kind="stable")
+        np.testing.assert_array_equal(sk.view(np.int64), ki[order])
+    with pytest.raises(_lib.B40CError):
+        en.set_policy(12, 4)
+    with pytest.raises(_lib.B40CError):
+        en.set_policy(8, 9)
+    en.close()
+
+
+def test_sort_autotuner_small(rs):
+    """The sort tuner times every policy on a small input, checks each
+    output against numpy's stable sort, and returns one of them."""
+    from back40computing_amd import autotune
+    space = [(8, 4), (16, 2)]
+    best, ms, hist = autotune.tune_sort(200003, "u64", False, repeats=1, space=space)
+    assert best in space and ms > 0
+    assert all(t is not None for _, t in hist)
+    best, ms, hist = autotune.tune_sort(100001, "f32", True, repeats=1, space=space)
+    assert best in space and all(t is not None for _, t in hist)
+
+
 @pytest.mark.parametrize("scale,ef,seed", [(6, 4, 1), (10, 16, 0), (14, 16, 5), (16, 16, 0), (22, 16, 0)])
 def test_rmat_csr_bit_exact(G, orc, scale, ef, seed):
     print("rmat scale=%d ef=%d seed=%d" % (scale, ef, seed), flush=True)
