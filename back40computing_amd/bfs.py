@@ -285,7 +285,7 @@ class EnactorHybrid(_Enactor):
 
 
 class EnactorPersistent(_Enactor):
-    """Expand-contract levels in one cooperative launch with a grid barrier."""
+    """Expand-contract levels in one launch with a software grid barrier."""
     STRATEGY = _lib.PERSISTENT
 
 

@@ -955,7 +955,7 @@ __device__ void heavy_body(BfsArgs a, int ph, int32_t depth_next, int slot, type
     stats_end(t0, a, slot);
 }
 
-// Persistent kernels: all levels in one cooperative launch with a software
+// Persistent kernels: all levels in one launch of a co-resident grid with a software
 // grid barrier between phases (contract_expand_atomic::KernelGlobalBarrier,
 // kernel.cuh:281-554; two_phase::Kernel, two_phase/kernel.cuh:60-313).
 // The loop control (the input frontier's length) is read by one thread per

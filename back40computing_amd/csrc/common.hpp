@@ -138,7 +138,7 @@ __device__ __forceinline__ int wg_exclusive_scan(int x, int& total, int* s_wave)
 }
 
 // ---------------------------------------------------------------------------
-// Software grid barrier for cooperatively launched (co-resident) grids
+// Software grid barrier for grids whose workgroups are all co-resident
 // (replaces GlobalBarrier, b40c/util/global_barrier.cuh:47-127).
 //
 // Each workgroup except 0 owns one arrival word and stores the barrier's
@@ -169,7 +169,7 @@ __device__ __forceinline__ int wg_exclusive_scan(int x, int& total, int* s_wave)
 // of hanging.  grid_sync returns false in every thread of every workgroup
 // that saw the abort; callers stop their level loop then.  After an abort
 // the epochs of the workgroups disagree, so the host clears the barrier
-// memory before its next cooperative launch.
+// memory before its next persistent launch.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kBarrierSpinLimit = 1u << 22;
 

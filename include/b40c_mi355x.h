@@ -324,8 +324,8 @@ void* b40c_msbfs_stream(b40c_msbfs* m);
 void b40c_msbfs_destroy(b40c_msbfs* m);
 
 /* ---- diagnostics ---------------------------------------------------------- */
-/* Times `iterations` back-to-back grid barriers of a cooperatively launched
- * grid of `grid_size` workgroups (test/gbarrier/test_gbarrier.cu:43-113).
+/* Times `iterations` back-to-back grid barriers of one launch of a
+ * co-resident grid of `grid_size` workgroups (test/gbarrier/test_gbarrier.cu:43-113).
  * Returns milliseconds per barrier in *ms_per_sync. */
 b40c_status b40c_gbarrier_benchmark(int32_t grid_size, int32_t iterations, double* ms_per_sync);
 /* Correctness check of the same barrier (the reference has none): every

@@ -162,17 +162,23 @@ def test_admission_policy_does_not_change_labels(mods, orc, pool):
     for gap in (1, 6):
         eng = batch.MultiSourceBFS(mg, label_budget_bytes=1 << 28)
         eng.set_admission(gap)
+        # Ids are reused once a result is collected, so each id maps to its
+        # query only until then.
         ids, out = {}, {}
+        submitted = 0
         for wave in arrivals:  # one graph's queries per level boundary
             for g, s in wave:
                 ids[eng.submit(g, s)] = (g, s)
+                submitted += 1
             eng.step()
             for q in eng.poll():
-                out[ids[q]] = eng.result(q, ids[q][0])[0]
+                key = ids.pop(q)
+                out[key] = eng.result(q, key[0])[0]
         eng.run()
         for q in eng.poll():
-            out[ids[q]] = eng.result(q, ids[q][0])[0]
-        assert len(out) == len(ids)
+            key = ids.pop(q)
+            out[key] = eng.result(q, key[0])[0]
+        assert len(out) == submitted and not ids
         results.append(out)
     for key, depth in results[0].items():
         np.testing.assert_array_equal(depth, results[1][key])

@@ -238,10 +238,17 @@ def test_autotuner_label_verifier(orc):
         if stranger is not None:
             bp = pred.copy(); bp[v] = stranger
             assert not verify_bfs_labels(off, cols, src, depth, bp, keys)
+            # The predecessor-only mode (depth already accepted) still
+            # catches a predecessor that is not a neighbour.
+            assert not verify_bfs_labels(off, cols, src, depth, bp, keys, depth_verified=True)
         same_level = [u for u in nbrs if depth[u] == depth[v]]
         if same_level:
             bp = pred.copy(); bp[v] = same_level[0]
             assert not verify_bfs_labels(off, cols, src, depth, bp, keys)
+            assert not verify_bfs_labels(off, cols, src, depth, bp, keys, depth_verified=True)
+        assert verify_bfs_labels(off, cols, src, depth, pred, keys, depth_verified=True)
+        bp = pred.copy(); bp[v] = -1
+        assert not verify_bfs_labels(off, cols, src, depth, bp, keys, depth_verified=True)
         bad, bp = depth.copy(), pred.copy()
         bad[v], bp[v] = -1, -1
         assert not verify_bfs_labels(off, cols, src, bad, bp, keys)
