@@ -160,8 +160,8 @@ SIGNATURES = {
     "b40c_enact_search": (_i32, [_vp, _vp, _i32, _i32, _i32]),
     "b40c_get_statistics": (_i32, [_vp, _p_i64, _p_i32, _p_d]),
     "b40c_enactor_destroy": (None, [_vp]),
-    "b40c_gbarrier_benchmark": (_i32, [_i32, _i32, _p_d]),
-    "b40c_gbarrier_check": (_i32, [_i32, _i32, _p_i64]),
+    "b40c_gbarrier_benchmark": (_i32, [_i32, _i32, _i32, _p_d]),
+    "b40c_gbarrier_check": (_i32, [_i32, _i32, _i32, _p_i64]),
     "b40c_microbench_search": (_i32, [_vp, _i32, _i32, _vp, ctypes.POINTER(MbLevel), _i32, _p_i32]),
     "b40c_multigraph_pack": (_i32, [_vp, _i32, _pp]),
     "b40c_multigraph_info": (_i32, [_vp, _p_i32, _vp, _vp]),

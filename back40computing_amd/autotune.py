@@ -209,15 +209,18 @@ class LabelsInvalid(AssertionError):
     pass
 
 
-def tune(graph, strategy, sources, rounds=2, repeats=2, log=None, knobs=None, space=None):
-    """Coordinate descent from the library defaults.  Returns (tuning, ms,
-    history).  The defaults' labels are verified as BFS labels first; every
+def tune(graph, strategy, sources, rounds=2, repeats=2, log=None, knobs=None, space=None, start=None):
+    """Coordinate descent from the library defaults (overridden by `start`,
+    e.g. an earlier table entry).  Returns (tuning, ms, history).  The
+    starting point's labels are verified as BFS labels first; every
     candidate must reproduce them."""
     from . import bfs
     space = space or SPACE
     en = bfs.ENACTORS[strategy]()
     best = {k: v for k, v in en.tuning.items() if k in space}
     en.close()
+    if start:
+        best.update({k: v for k, v in start.items() if k in space})
     off, cols = graph.to_host()
     keys = sorted_edge_keys(off, cols)
     best_ms, ref, ref_pred = measure(graph, strategy, sources, best, repeats)
@@ -392,6 +395,9 @@ def main(argv=None):
     ap.add_argument("--sources", type=int, default=8)
     ap.add_argument("--rounds", type=int, default=2)
     ap.add_argument("--out", default=TABLE_PATH)
+    ap.add_argument("--knobs", default=None, help="comma-separated knobs to search (default: the strategy's set)")
+    ap.add_argument("--from-table", action="store_true",
+                    help="start from the entry already recorded for the graph class, if any")
     ap.add_argument("--sort", default=None,
                     help="tune the radix sort instead: KEY[,pairs] (KEY in u32 i32 f32 u64 i64 f64), e.g. u64")
     ap.add_argument("--sort-n", type=int, default=1 << 27, help="keys per sort when tuning the radix sort")
@@ -413,9 +419,13 @@ def main(argv=None):
     off = np.empty(g.nodes + 1, np.int32)
     check(lib().b40c_graph_copy_to_host(g.handle, _lib.ptr(off), None))
     srcs = pick_sources(off, 7, args.sources)
+    knobs = args.knobs.split(",") if args.knobs else None
+    arch = bfs.device_info()["arch"]
     for s in (int(x) for x in args.strategies.split(",")):
         log("tuning strategy %d on %s (V=%d E=%d)" % (s, args.graph, g.nodes, g.edges))
-        best, ms = tune_and_record(g, s, srcs, path=args.out, log=log, rounds=args.rounds)
+        start = lookup(arch, s, g.nodes, g.edges) if args.from_table else None
+        best, ms = tune_and_record(g, s, srcs, path=args.out, log=log, rounds=args.rounds, knobs=knobs,
+                                   start=start)
         log("strategy %d best %.4f ms/search: %s" % (s, ms, best))
     return 0
 

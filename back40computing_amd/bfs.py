@@ -36,17 +36,19 @@ def device_info():
     return {"num_cus": cus.value, "arch": arch.value.decode()}
 
 
-def gbarrier_benchmark(grid_size, iterations):
-    """Milliseconds per software grid barrier (test/gbarrier/test_gbarrier.cu)."""
+def gbarrier_benchmark(grid_size, iterations, light=True):
+    """Milliseconds per software grid barrier (test/gbarrier/test_gbarrier.cu).
+    light: the persistent kernels' mode (data through agent-scope accesses,
+    no cache maintenance); False: agent-scope release/acquire per workgroup."""
     ms = ctypes.c_double()
-    check(lib().b40c_gbarrier_benchmark(grid_size, iterations, ctypes.byref(ms)))
+    check(lib().b40c_gbarrier_benchmark(grid_size, iterations, 1 if light else 0, ctypes.byref(ms)))
     return ms.value
 
 
-def gbarrier_check(grid_size, iterations):
+def gbarrier_check(grid_size, iterations, light=True):
     """Number of stale/wrong words seen across the software grid barrier."""
     bad = ctypes.c_int64()
-    check(lib().b40c_gbarrier_check(grid_size, iterations, ctypes.byref(bad)))
+    check(lib().b40c_gbarrier_check(grid_size, iterations, 1 if light else 0, ctypes.byref(bad)))
     return bad.value
 
 

@@ -63,8 +63,39 @@ struct BfsArgs {
     int32_t td_grid;         // direction-optimizing: workgroups of a top-down step
     const uint32_t* iso;     // symmetric graphs: isolated-vertex bitmap, pre-marked visited (else null)
     int32_t dev_base;        // 1: phase = ctr->ph_base + kernel argument (captured, replayed level loops)
+    int32_t coherent;        // persistent launches: frontier data through agent-scope accesses (see coh_st)
     long long edges;
 };
+
+// Frontier data that other workgroups of the same persistent launch read
+// after a grid barrier (queue entries, grid-tier entries and chunk maps).
+// With `coh` the accesses are agent-scope atomic ones, which the GPU keeps
+// coherent across XCDs without any cache maintenance, so the persistent
+// kernels' barrier only has to wait for each wave's stores to complete
+// (grid_sync's light mode) instead of writing back and invalidating its
+// XCD's L2 once per workgroup.  Without `coh` (separate launches) they are
+// plain accesses.
+__device__ __forceinline__ void coh_st(int32_t* p, int32_t v, bool coh) {
+    if (coh) __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else *p = v;
+}
+__device__ __forceinline__ int32_t coh_ld(const int32_t* p, bool coh) {
+    return coh ? __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : *p;
+}
+__device__ __forceinline__ void coh_st2(int2* p, int2 v, bool coh) {
+    if (coh) {
+        const unsigned long long w = (unsigned long long)(uint32_t)v.x | ((unsigned long long)(uint32_t)v.y << 32);
+        __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        *p = v;
+    }
+}
+__device__ __forceinline__ int2 coh_ld2(const int2* p, bool coh) {
+    if (!coh) return *p;
+    const unsigned long long w = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+    return make_int2((int32_t)(uint32_t)w, (int32_t)(uint32_t)(w >> 32));
+}
 
 constexpr int HEAVY_CHUNK = 256;     // grid-tier edges per work item (one per thread)
 constexpr int HEAVY_ROUND = 4;       // grid-tier work items a workgroup has in flight at once
@@ -94,6 +125,7 @@ struct GlobalQueue {
     uint32_t* len;
     uint32_t cap;
     uint32_t* overflow;
+    bool coh = false;  // entries written with agent-scope stores (persistent launches)
 };
 
 // Wave-uniform call: every lane of the wave must be active.
@@ -123,8 +155,8 @@ __device__ __forceinline__ void lq_append(LdsQueueT<VCAP>& q, const GlobalQueue&
         if (want && pos >= (uint32_t)LQ_CAP) {
             const uint32_t gp = gb + (pos - spill_start);
             if (gp < g.cap) {
-                g.keys[gp] = k;
-                if (VALS) g.vals[gp] = v;
+                coh_st(&g.keys[gp], k, g.coh);
+                if (VALS) coh_st(&g.vals[gp], v, g.coh);
             } else {
                 st_relaxed(g.overflow, 1u);
             }
@@ -148,8 +180,8 @@ __device__ __forceinline__ void lq_flush(LdsQueueT<VCAP>& q, const GlobalQueue& 
     for (uint32_t j = threadIdx.x; j < cnt; j += BFS_THREADS) {
         const uint32_t gp = gb + j;
         if (gp < g.cap) {
-            g.keys[gp] = q.keys[j];
-            if (VALS) g.vals[gp] = q.vals[j];
+            coh_st(&g.keys[gp], q.keys[j], g.coh);
+            if (VALS) coh_st(&g.vals[gp], q.vals[j], g.coh);
         }
     }
     if (threadIdx.x == 0 && gb + cnt > g.cap) st_relaxed(g.overflow, 1u);

@@ -91,6 +91,7 @@ __device__ __forceinline__ GlobalQueue out_queue(const BfsArgs& a, int ph) {
     q.len = &a.ctr->qlen[(ph + 1) & 3];
     q.cap = a.qcap[(ph + 1) & 1];
     q.overflow = &a.ctr->overflow;
+    q.coh = a.coherent != 0;
     return q;
 }
 
@@ -134,8 +135,9 @@ __device__ __forceinline__ void defer_heavy(const BfsArgs& a, int ph, int32_t v,
         const uint32_t entry = base_hi + (uint32_t)lane_prefix(m);
         const uint32_t c0 = base_lo + ch_incl - nch;
         if (entry < a.heavy_cap && c0 + nch <= a.chunk_cap) {
-            a.heavy[entry] = make_int2(v, (int)c0);
-            for (uint32_t c = 0; c < nch; ++c) a.chunk_map[c0 + c] = (int32_t)entry;
+            const bool coh = a.coherent != 0;
+            coh_st2(&a.heavy[entry], make_int2(v, (int)c0), coh);
+            for (uint32_t c = 0; c < nch; ++c) coh_st(&a.chunk_map[c0 + c], (int32_t)entry, coh);
         } else {
             st_relaxed(&a.ctr->overflow, 1u);
         }
@@ -202,7 +204,7 @@ __device__ void td_phase(const BfsArgs& a, int ph, int32_t depth_next, uint32_t*
         const uint32_t i = base + threadIdx.x;
         int32_t u = -1, start = 0, deg = 0;
         if (i < n_in) {
-            u = qin[i];
+            u = coh_ld(&qin[i], a.coherent != 0);
             if (valid_vertex(a, u)) {
                 start = a.offs[u];
                 deg = a.offs[u + 1] - start;
@@ -233,8 +235,8 @@ __device__ void ce_phase(const BfsArgs& a, int ph, int32_t depth_here, LevelShar
         const uint32_t i = base + threadIdx.x;
         int32_t v = -1, start = 0, deg = 0;
         if (i < n_in) {
-            v = qk[i];
-            const int32_t p = qv[i];
+            v = coh_ld(&qk[i], a.coherent != 0);
+            const int32_t p = coh_ld(&qv[i], a.coherent != 0);
             bool keep = valid_vertex(a, v);
             if (keep) keep = !visited_test(a.visited, v);
             if (keep) keep = !history_seen(s.hist, v);
@@ -270,8 +272,8 @@ __device__ void tp_contract_phase(const BfsArgs& a, int ph, int32_t depth_here, 
         bool keep = false;
         int32_t v = -1;
         if (i < n_in) {
-            v = qk[i];
-            const int32_t p = qv[i];
+            v = coh_ld(&qk[i], a.coherent != 0);
+            const int32_t p = coh_ld(&qv[i], a.coherent != 0);
             keep = valid_vertex(a, v);
             if (keep) keep = !visited_test(a.visited, v);
             if (keep) keep = !history_seen(s.hist, v);
@@ -342,8 +344,8 @@ __device__ __forceinline__ void expand_tile_in_place(const BfsArgs& a, const Glo
             if (col[r] < 0) continue;
             const uint64_t pos = (uint64_t)base + (uint32_t)(p0 + r * BFS_THREADS + tid);
             if (pos < out.cap) {
-                out.keys[pos] = v[r];
-                out.vals[pos] = src[r];
+                coh_st(&out.keys[pos], v[r], out.coh);
+                coh_st(&out.vals[pos], src[r], out.coh);
             }
         }
     }
@@ -362,7 +364,7 @@ __device__ void tp_expand_phase(const BfsArgs& a, int ph, LevelShared& s, uint32
         const uint32_t i = base + threadIdx.x;
         int32_t u = -1, start = 0, deg = 0;
         if (i < n_in) {
-            u = qin[i];
+            u = coh_ld(&qin[i], a.coherent != 0);
             if (valid_vertex(a, u)) {
                 start = a.offs[u];
                 deg = a.offs[u + 1] - start;
@@ -386,8 +388,8 @@ __device__ void tp_filter_phase(const BfsArgs& a, int ph, LevelShared& s) {
         bool keep = false;
         int32_t v = -1, p = 0;
         if (i < n_in) {
-            v = qk[i];
-            p = qv[i];
+            v = coh_ld(&qk[i], a.coherent != 0);
+            p = coh_ld(&qv[i], a.coherent != 0);
             keep = valid_vertex(a, v);
             if (keep) keep = !visited_test(a.visited, v);
             if (keep) keep = !history_seen(s.hist, v);
@@ -832,11 +834,12 @@ __device__ void heavy_body(BfsArgs a, int ph, int32_t depth_next, int slot, type
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const uint32_t c = c0 + (uint32_t)r * stride;
-            entry[r] = c < C ? a.chunk_map[c] : -1;
+            entry[r] = c < C ? coh_ld(&a.chunk_map[c], a.coherent != 0) : -1;
         }
         int2 eu[R];
 #pragma unroll
-        for (int r = 0; r < R; ++r) eu[r] = (uint32_t)entry[r] < H ? a.heavy[entry[r]] : make_int2(-1, 0);
+        for (int r = 0; r < R; ++r)
+            eu[r] = (uint32_t)entry[r] < H ? coh_ld2(&a.heavy[entry[r]], a.coherent != 0) : make_int2(-1, 0);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const uint32_t c = c0 + (uint32_t)r * stride;
@@ -882,9 +885,9 @@ __device__ void heavy_body(BfsArgs a, int ph, int32_t depth_next, int slot, type
                 int32_t u = -1, start = 0, len = 0;
                 if (i < nb) {
                     const uint32_t c = blockIdx.x + (b0 + i) * stride;
-                    const int32_t entry = a.chunk_map[c];
+                    const int32_t entry = coh_ld(&a.chunk_map[c], a.coherent != 0);
                     if ((uint32_t)entry < H) {
-                        const int2 eu = a.heavy[entry];
+                        const int2 eu = coh_ld2(&a.heavy[entry], a.coherent != 0);
                         if (valid_vertex(a, eu.x) && (uint32_t)eu.y <= c) {
                             const int32_t s0 = a.offs[eu.x], deg = a.offs[eu.x + 1] - s0;
                             const long long off = (long long)(c - (uint32_t)eu.y) * HEAVY_CHUNK;
@@ -932,8 +935,8 @@ __device__ void heavy_body(BfsArgs a, int ph, int32_t depth_next, int slot, type
                     const uint32_t i = i0 + r;
                     const uint64_t pos = (uint64_t)base + s.off[i] + (uint32_t)tid;
                     if (pos < out.cap) {
-                        out.keys[pos] = v[r];
-                        out.vals[pos] = s.u[i];
+                        coh_st(&out.keys[pos], v[r], out.coh);
+                        coh_st(&out.vals[pos], s.u[i], out.coh);
                     }
                 }
             }
@@ -983,7 +986,7 @@ __device__ void fused_body(BfsArgs a, int ph, int32_t depth, int max_levels, uin
     unsigned long long t0 = 0, busy = 0;
     if (a.wg_stats && threadIdx.x == 0) t0 = wg_clock();
     shared_init(s);
-    const GridBarrier bar{a.bar_arrive, &a.ctr->bar_release, &a.ctr->bar_abort};
+    const GridBarrier bar{a.bar_arrive, &a.ctr->bar_release, &a.ctr->bar_abort, (uint32_t)a.coherent};
     uint32_t epoch = barrier_base_epoch(bar);
     if (blockIdx.x == 0 && threadIdx.x == 0) a.ctr->saturated = 0;
     // Barrier payload: the length of the frontier queue in counter slot
@@ -1082,12 +1085,18 @@ __device__ void gbarrier_check_body(const GridBarrier& bar, uint32_t* data, int 
     const uint32_t G = gridDim.x, b = blockIdx.x;
     uint32_t epoch = barrier_base_epoch(bar);
     unsigned long long bad = 0;
+    const bool coh = bar.light != 0;  // light barrier: the slabs go through agent-scope accesses
     for (int it = 0; it < iterations; ++it) {
-        for (int j = threadIdx.x; j < words; j += BFS_THREADS) data[(size_t)b * words + j] = gb_pattern(it, b, j);
+        for (int j = threadIdx.x; j < words; j += BFS_THREADS)
+            coh_st((int32_t*)&data[(size_t)b * words + j], (int32_t)gb_pattern(it, b, j), coh);
         // The payload is the iteration number as workgroup 0 saw it in a
         // word every workgroup wrote before arriving: all must agree.
         uint32_t seen = 0;
-        if (!grid_sync(bar, ++epoch, [&] { return data[(size_t)(G - 1) * words] ^ gb_pattern(it, G - 1, 0) ^ (uint32_t)it; },
+        if (!grid_sync(bar, ++epoch,
+                       [&] {
+                           return (uint32_t)coh_ld((const int32_t*)&data[(size_t)(G - 1) * words], coh) ^
+                                  gb_pattern(it, G - 1, 0) ^ (uint32_t)it;
+                       },
                        s_word, seen))
             break;
         bad += (seen != (uint32_t)it) ? 1u : 0u;
@@ -1095,7 +1104,8 @@ __device__ void gbarrier_check_body(const GridBarrier& bar, uint32_t* data, int 
         for (int k = 0; k < 3; ++k) {
             const uint32_t o = others[k];
             for (int j = threadIdx.x; j < words; j += BFS_THREADS)
-                bad += (data[(size_t)o * words + j] != gb_pattern(it, o, j)) ? 1u : 0u;
+                bad += ((uint32_t)coh_ld((const int32_t*)&data[(size_t)o * words + j], coh) != gb_pattern(it, o, j))
+                           ? 1u : 0u;
         }
         if (!grid_sync(bar, ++epoch, s_word)) break;
     }

@@ -177,6 +177,13 @@ struct GridBarrier {
     uint32_t* arrive;              // [grid] arrival epoch of each workgroup (word 0 unused)
     unsigned long long* release;   // (payload << 32) | epoch of the last completed barrier
     uint32_t* abort_flag;          // nonzero once a wait ran out of polls
+    // Light mode: every piece of data the grid passes between workgroups
+    // across this barrier is written and read with agent-scope atomic
+    // accesses, which stay coherent across XCDs without cache maintenance.
+    // Arrival then only waits for each wave's stores to complete; the
+    // agent-scope release and acquire (an L2 writeback and an L2
+    // invalidate in every workgroup, serialised per XCD) are left out.
+    uint32_t light;
 };
 
 __device__ __forceinline__ uint32_t barrier_base_epoch(const GridBarrier& b) {
@@ -190,6 +197,14 @@ __device__ __forceinline__ uint32_t barrier_base_epoch(const GridBarrier& b) {
 template <typename PayloadFn>
 __device__ __forceinline__ bool grid_sync(const GridBarrier& b, uint32_t epoch, PayloadFn make_payload,
                                           uint32_t* s_word, uint32_t& payload) {
+    const bool light = b.light != 0;
+    if (light) {
+        // Every wave's agent-scope stores have completed (and the compiler
+        // moves no memory access across this point).
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_waitcnt(0);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    }
     __syncthreads();
     bool aborted = false;
     const uint32_t nb = gridDim.x;
@@ -206,16 +221,24 @@ __device__ __forceinline__ bool grid_sync(const GridBarrier& b, uint32_t epoch, 
             }
             __builtin_amdgcn_s_sleep(1);
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (!light) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         aborted = __syncthreads_or(aborted) != 0;
         if (threadIdx.x == 0 && !aborted) {
             const uint32_t p = make_payload();
             *s_word = p;
-            __hip_atomic_store(b.release, ((unsigned long long)p << 32) | epoch, __ATOMIC_RELEASE,
-                               __HIP_MEMORY_SCOPE_AGENT);
+            const unsigned long long rw = ((unsigned long long)p << 32) | epoch;
+            if (light) {
+                __atomic_signal_fence(__ATOMIC_SEQ_CST);
+                __hip_atomic_store(b.release, rw, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                __hip_atomic_store(b.release, rw, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            }
         }
     } else if (threadIdx.x == 0) {
-        __hip_atomic_store(&b.arrive[blockIdx.x], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (light)
+            __hip_atomic_store(&b.arrive[blockIdx.x], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        else
+            __hip_atomic_store(&b.arrive[blockIdx.x], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         uint32_t spins = 0;
         unsigned long long r;
         while ((uint32_t)(r = __hip_atomic_load(b.release, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != epoch) {
@@ -226,9 +249,10 @@ __device__ __forceinline__ bool grid_sync(const GridBarrier& b, uint32_t epoch, 
             }
             __builtin_amdgcn_s_sleep(1);
         }
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (!light) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         *s_word = (uint32_t)(r >> 32);
     }
+    if (light) __atomic_signal_fence(__ATOMIC_SEQ_CST);
     const bool ok = __syncthreads_or(aborted) == 0;
     payload = *s_word;
     __syncthreads();  // s_word may be rewritten by the next barrier

@@ -326,12 +326,16 @@ void b40c_msbfs_destroy(b40c_msbfs* m);
 /* ---- diagnostics ---------------------------------------------------------- */
 /* Times `iterations` back-to-back grid barriers of one launch of a
  * co-resident grid of `grid_size` workgroups (test/gbarrier/test_gbarrier.cu:43-113).
- * Returns milliseconds per barrier in *ms_per_sync. */
-b40c_status b40c_gbarrier_benchmark(int32_t grid_size, int32_t iterations, double* ms_per_sync);
+ * light = 0: agent-scope release/acquire at every workgroup (any data);
+ * light = 1: the persistent kernels' mode (data passed with agent-scope
+ * accesses, no cache maintenance).  Returns milliseconds per barrier in
+ * *ms_per_sync. */
+b40c_status b40c_gbarrier_benchmark(int32_t grid_size, int32_t iterations, int32_t light, double* ms_per_sync);
 /* Correctness check of the same barrier (the reference has none): every
  * workgroup writes a slab, passes the barrier and compares three other
- * workgroups' slabs; *mismatches counts stale or wrong words. */
-b40c_status b40c_gbarrier_check(int32_t grid_size, int32_t iterations, int64_t* mismatches);
+ * workgroups' slabs (through agent-scope accesses when light = 1);
+ * *mismatches counts stale or wrong words. */
+b40c_status b40c_gbarrier_check(int32_t grid_size, int32_t iterations, int32_t light, int64_t* mismatches);
 
 /* ---- per-stage BFS microbenchmarks ----------------------------------------
  * A level-synchronous BFS split into its status lookup (edge frontier ->

@@ -292,6 +292,9 @@ inline uint32_t __builtin_amdgcn_mbcnt_hi(uint32_t mask, uint32_t add) {
 #define __hip_atomic_fetch_max(p, v, order, scope) emu::fetch_max((p), (v))
 #define __hip_atomic_fetch_min(p, v, order, scope) emu::fetch_min((p), (v))
 #define __builtin_amdgcn_fence(order, scope) __atomic_thread_fence(__ATOMIC_SEQ_CST)
+// A wave's wait for its outstanding memory operations: host memory accesses
+// have completed when they return, so a full fence stands in for it.
+inline void __builtin_amdgcn_s_waitcnt(int) { __atomic_thread_fence(__ATOMIC_SEQ_CST); }
 
 template <class T>
 inline T atomicAdd(T* p, T v) {

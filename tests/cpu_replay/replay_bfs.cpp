@@ -13,7 +13,7 @@
 // sanitizer report or as wrong labels.
 //
 // usage: replay_bfs GRAPH SRC STRATEGY MODE GRID CTA_TH WAVE_TH GRID_TH SAT_QUIT QUEUE_SIZING LDS_FILL OUT
-//        replay_bfs --gbarrier GRID ITERATIONS
+//        replay_bfs --gbarrier GRID ITERATIONS LIGHT
 //   GRAPH: int32 n, int64 m, int32 offsets[n+1], int32 cols[m] (little endian)
 //   STRATEGY/MODE: the library's numbering (0 EC, 1 CE, 2 TP, 3 hybrid, 4 persistent; 0 default, 1 fused, 2 iterative)
 //   LDS_FILL: byte value every LDS block starts with
@@ -181,6 +181,7 @@ struct Replay {
     void fused(int kind, int ph, int32_t depth_arg, uint32_t sat) {
         BfsArgs fa = a;
         if (!heavy_needed) fa.grid_th = INT_MAX;
+        fa.coherent = 1;  // as the library: light barrier, frontier data through agent-scope accesses
         if (kind == FK_TD)
             run<FusedLds<FK_TD>>(grid, [&](FusedLds<FK_TD>& l) {
                 fused_body<FK_TD>(fa, ph, depth_arg, 1 << 30, sat, l.s, l.hs);
@@ -215,7 +216,7 @@ std::vector<char> read_file(const char* path) {
 
 // Grid-barrier visibility check (the library's gbarrier_check_body) on
 // `grid` workgroups for `iters` iterations; prints the mismatch count.
-int gbarrier_check(unsigned grid, int iters) {
+int gbarrier_check(unsigned grid, int iters, int light) {
     const int words = 256;
     DevBuf data, bad, arrive, rel, abort_flag;
     data.alloc((size_t)grid * words * 4, 0);
@@ -223,7 +224,8 @@ int gbarrier_check(unsigned grid, int iters) {
     arrive.alloc((size_t)grid * 4, 0);
     rel.alloc(8, 0);
     abort_flag.alloc(4, 0);
-    const GridBarrier bar{arrive.as<uint32_t>(), rel.as<unsigned long long>(), abort_flag.as<uint32_t>()};
+    const GridBarrier bar{arrive.as<uint32_t>(), rel.as<unsigned long long>(), abort_flag.as<uint32_t>(),
+                          (uint32_t)light};
     for (int round = 0; round < 2; ++round)  // a second launch continues the epochs of the first
         run<uint32_t>(grid, [&](uint32_t& s_word) {
             gbarrier_check_body(bar, data.as<uint32_t>(), words, iters, bad.as<unsigned long long>(), &s_word);
@@ -234,7 +236,8 @@ int gbarrier_check(unsigned grid, int iters) {
 }
 
 int main(int argc, char** argv) {
-    if (argc == 4 && std::string(argv[1]) == "--gbarrier") return gbarrier_check((unsigned)std::atoi(argv[2]), std::atoi(argv[3]));
+    if (argc == 5 && std::string(argv[1]) == "--gbarrier")
+        return gbarrier_check((unsigned)std::atoi(argv[2]), std::atoi(argv[3]), std::atoi(argv[4]));
     if (argc != 13) {
         std::fprintf(stderr,
                      "usage: %s GRAPH SRC STRATEGY MODE GRID CTA_TH WAVE_TH GRID_TH SAT_QUIT QUEUE_SIZING LDS_FILL "

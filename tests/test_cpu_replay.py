@@ -149,12 +149,14 @@ def test_replay_forced_overflow(replay_exe, workdir, rmat10, sm):
 
 
 @pytest.mark.parametrize("grid", [2, 5])
-def test_replay_grid_barrier(replay_exe, grid):
+@pytest.mark.parametrize("light", [1, 0])
+def test_replay_grid_barrier(replay_exe, grid, light):
     """The grid barrier's visibility check (words written before a barrier
     are seen after it, and the payload workgroup 0 publishes is the same in
-    every workgroup), over two launches that continue one epoch sequence."""
+    every workgroup), over two launches that continue one epoch sequence,
+    in both barrier modes."""
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0")
-    r = subprocess.run([replay_exe, "--gbarrier", str(grid), "30"], capture_output=True, text=True, env=env,
-                       timeout=300)
+    r = subprocess.run([replay_exe, "--gbarrier", str(grid), "30", str(light)], capture_output=True, text=True,
+                       env=env, timeout=300)
     assert r.returncode == 0, r.stderr[-3000:]
     assert r.stdout.split() == ["mismatches", "0", "aborted", "0", "epoch", str(2 * 30 * 2)]

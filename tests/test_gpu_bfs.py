@@ -211,21 +211,25 @@ def test_fused_entry_points(bfsmod, orc, rmat16):
             check(orc, off, cols, s, depth, pred, {"search_depth": sd})
 
 
-def test_gbarrier_zz_visibility(bfsmod):
+@pytest.mark.parametrize("light", [True, False], ids=["light", "fenced"])
+def test_gbarrier_zz_visibility(bfsmod, light):
     """Words written before the software grid barrier are seen by other
-    workgroups after it (indices are fixed and in bounds; values are only
-    compared)."""
+    workgroups after it, in both barrier modes (light: the words go through
+    agent-scope accesses; fenced: plain accesses ordered by the barrier's
+    release/acquire).  Indices are fixed and in bounds; values are only
+    compared."""
     info = bfsmod.device_info()
-    for grid in (info["num_cus"], 2 * info["num_cus"], 6 * info["num_cus"]):
-        print("gbarrier_check grid=%d" % grid, flush=True)
-        assert bfsmod.gbarrier_check(grid, 200) == 0
+    for grid in (info["num_cus"] // 4, info["num_cus"], 2 * info["num_cus"], 6 * info["num_cus"]):
+        print("gbarrier_check grid=%d light=%s" % (grid, light), flush=True)
+        assert bfsmod.gbarrier_check(grid, 200, light) == 0
 
 
-def test_gbarrier_zz_benchmark(bfsmod):
+@pytest.mark.parametrize("light", [True, False], ids=["light", "fenced"])
+def test_gbarrier_zz_benchmark(bfsmod, light):
     info = bfsmod.device_info()
-    for grid in (info["num_cus"], 2 * info["num_cus"], 6 * info["num_cus"]):
-        ms = bfsmod.gbarrier_benchmark(grid, 2000)
-        print("gbarrier %.2f us/sync at grid %d" % (ms * 1e3, grid), flush=True)
+    for grid in (info["num_cus"] // 4, info["num_cus"], 2 * info["num_cus"], 6 * info["num_cus"]):
+        ms = bfsmod.gbarrier_benchmark(grid, 2000, light)
+        print("gbarrier %.2f us/sync at grid %d light=%s" % (ms * 1e3, grid, light), flush=True)
         assert 0 < ms < 1.0
 
 
