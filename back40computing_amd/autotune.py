@@ -255,10 +255,11 @@ def tune(graph, strategy, sources, rounds=2, repeats=2, log=None, knobs=None, sp
                         log("  REJECTED %s=%s: %s" % (knob, val, ex))
                     continue
                 history.append((dict(cand), ms))
-                if ms < best_ms * 0.98:
+                better = ms < best_ms * 0.98
+                if log:
+                    log("  %s=%s -> %.4f ms/search%s" % (knob, val, ms, " (best)" if better else ""))
+                if better:
                     best, best_ms, improved = cand, ms, True
-                    if log:
-                        log("  %s=%s -> %.4f ms/search" % (knob, val, ms))
         if not improved:
             break
     return best, best_ms, history

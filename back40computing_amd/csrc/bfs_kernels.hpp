@@ -84,6 +84,22 @@ __device__ __forceinline__ void phase_begin(const BfsArgs& a, int ph, uint32_t n
     }
 }
 
+// Frontier entries per workgroup per step of a gathering phase.  Level
+// launches take full tiles of BFS_THREADS.  A persistent launch (a.coherent)
+// spreads a frontier smaller than its grid's tiles evenly over all its
+// workgroups: every level of a high-diameter graph is such a frontier, and
+// spread out each workgroup gathers its few lists' edges in one round of
+// the fine tier instead of a few workgroups gathering full tiles in several
+// rounds, each a dependent chain (column, visited word, claiming atomic).
+// Level launches run four times as many workgroups, and there the extra
+// workgroups' same-address queue and depth atomics cost more than the
+// rounds saved (200^3 lattice: 25 instead of 17 us per level).
+__device__ __forceinline__ uint32_t entries_per_wg(const BfsArgs& a, uint32_t n_in, uint32_t nblocks) {
+    if (!a.coherent || (uint64_t)n_in >= (uint64_t)nblocks * BFS_THREADS) return BFS_THREADS;
+    const uint32_t per = (n_in + nblocks - 1) / nblocks;
+    return per < 1u ? 1u : per;
+}
+
 __device__ __forceinline__ GlobalQueue out_queue(const BfsArgs& a, int ph) {
     GlobalQueue q;
     q.keys = a.qk[(ph + 1) & 1];
@@ -178,10 +194,16 @@ struct PairSink {
     }
 };
 
+// The deepest label: one atomic per workgroup that found a vertex.  A
+// persistent launch (a.coherent) leaves it to workgroup 0, which learns
+// after the level's barrier whether the next frontier is empty (every
+// vertex a top-down step claims is enqueued), so the grid's workgroups do
+// not all hit one counter at every level.
 template <int VCAP>
 __device__ __forceinline__ void td_finish(const BfsArgs& a, int ph, TdSink<VCAP>& sink, unsigned long long* red) {
     lq_flush<false>(sink.q, sink.out, 1);
     if (sink.count_edges) wg_add_u64(sink.eacc, &a.ctr->level_edges[(ph + 1) & 3], red);
+    if (a.coherent) return;
     if (__syncthreads_or(sink.found) && threadIdx.x == 0) atomic_max_u32(a.max_depth, (uint32_t)sink.depth_next);
 }
 
@@ -200,10 +222,11 @@ __device__ void td_phase(const BfsArgs& a, int ph, int32_t depth_next, uint32_t*
     const int32_t* qin = a.qk[ph & 1];
     const GlobalQueue out = out_queue(a, ph);
     TdSink<VCAP> sink{a, q, out, next_bm, depth_next, count_edges, false, 0ull};
-    for (uint32_t base = blockIdx.x * BFS_THREADS; base < n_in; base += nblocks * BFS_THREADS) {
+    const uint32_t per = entries_per_wg(a, n_in, nblocks);
+    for (uint32_t base = blockIdx.x * per; base < n_in; base += nblocks * per) {
         const uint32_t i = base + threadIdx.x;
         int32_t u = -1, start = 0, deg = 0;
-        if (i < n_in) {
+        if (threadIdx.x < per && i < n_in) {
             u = coh_ld(&qin[i], a.coherent != 0);
             if (valid_vertex(a, u)) {
                 start = a.offs[u];
@@ -231,10 +254,11 @@ __device__ void ce_phase(const BfsArgs& a, int ph, int32_t depth_here, LevelShar
     const GlobalQueue out = out_queue(a, ph);
     PairSink sink{s, out};
     bool found = false;
-    for (uint32_t base = blockIdx.x * BFS_THREADS; base < n_in; base += gridDim.x * BFS_THREADS) {
+    const uint32_t per = entries_per_wg(a, n_in, gridDim.x);
+    for (uint32_t base = blockIdx.x * per; base < n_in; base += gridDim.x * per) {
         const uint32_t i = base + threadIdx.x;
         int32_t v = -1, start = 0, deg = 0;
-        if (i < n_in) {
+        if (threadIdx.x < per && i < n_in) {
             v = coh_ld(&qk[i], a.coherent != 0);
             const int32_t p = coh_ld(&qv[i], a.coherent != 0);
             bool keep = valid_vertex(a, v);
@@ -1049,6 +1073,10 @@ __device__ void fused_body(BfsArgs a, int ph, int32_t depth, int max_levels, uin
             if (!grid_sync(bar, ++epoch, payload_for(ph, -1), &s.ctl, word)) break;
         }
         n_in = word & 0x7FFFFFFFu;
+        // Top-down: the level claimed vertices (at `depth`) iff the next
+        // frontier is not empty (td_finish leaves this to workgroup 0).
+        if (KIND == FK_TD && a.coherent && n_in != 0 && blockIdx.x == 0 && threadIdx.x == 0)
+            atomic_max_u32(a.max_depth, (uint32_t)depth);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         a.ctr->levels_run = (uint32_t)level;
