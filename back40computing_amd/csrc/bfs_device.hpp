@@ -63,7 +63,10 @@ struct BfsArgs {
     int32_t td_grid;         // direction-optimizing: workgroups of a top-down step
     const uint32_t* iso;     // symmetric graphs: isolated-vertex bitmap, pre-marked visited (else null)
     int32_t dev_base;        // 1: phase = ctr->ph_base + kernel argument (captured, replayed level loops)
-    int32_t coherent;        // persistent launches: frontier data through agent-scope accesses (see coh_st)
+    int32_t persistent;      // 1 inside a persistent (single-launch) search
+    int32_t coh_in;          // persistent: this phase's input frontier was written with coh_st
+    int32_t coh_out;         // persistent: this phase writes its output (queue, grid-tier entries) with coh_st
+    uint32_t light_limit;    // persistent: phases with at most this many input entries use the light barrier
     long long edges;
 };
 

@@ -85,9 +85,8 @@ __device__ __forceinline__ void phase_begin(const BfsArgs& a, int ph, uint32_t n
 }
 
 // Frontier entries per workgroup per step of a gathering phase.  Level
-// launches take full tiles of BFS_THREADS.  A persistent launch (a.coherent)
-// spreads a frontier smaller than its grid's tiles evenly over all its
-// workgroups: every level of a high-diameter graph is such a frontier, and
+// launches take full tiles of BFS_THREADS.  A persistent launch spreads a
+// frontier smaller than its grid's tiles evenly over all its workgroups: every level of a high-diameter graph is such a frontier, and
 // spread out each workgroup gathers its few lists' edges in one round of
 // the fine tier instead of a few workgroups gathering full tiles in several
 // rounds, each a dependent chain (column, visited word, claiming atomic).
@@ -95,7 +94,7 @@ __device__ __forceinline__ void phase_begin(const BfsArgs& a, int ph, uint32_t n
 // workgroups' same-address queue and depth atomics cost more than the
 // rounds saved (200^3 lattice: 25 instead of 17 us per level).
 __device__ __forceinline__ uint32_t entries_per_wg(const BfsArgs& a, uint32_t n_in, uint32_t nblocks) {
-    if (!a.coherent || (uint64_t)n_in >= (uint64_t)nblocks * BFS_THREADS) return BFS_THREADS;
+    if (!a.persistent || (uint64_t)n_in >= (uint64_t)nblocks * BFS_THREADS) return BFS_THREADS;
     const uint32_t per = (n_in + nblocks - 1) / nblocks;
     return per < 1u ? 1u : per;
 }
@@ -107,7 +106,7 @@ __device__ __forceinline__ GlobalQueue out_queue(const BfsArgs& a, int ph) {
     q.len = &a.ctr->qlen[(ph + 1) & 3];
     q.cap = a.qcap[(ph + 1) & 1];
     q.overflow = &a.ctr->overflow;
-    q.coh = a.coherent != 0;
+    q.coh = a.coh_out != 0;
     return q;
 }
 
@@ -151,7 +150,7 @@ __device__ __forceinline__ void defer_heavy(const BfsArgs& a, int ph, int32_t v,
         const uint32_t entry = base_hi + (uint32_t)lane_prefix(m);
         const uint32_t c0 = base_lo + ch_incl - nch;
         if (entry < a.heavy_cap && c0 + nch <= a.chunk_cap) {
-            const bool coh = a.coherent != 0;
+            const bool coh = a.coh_out != 0;
             coh_st2(&a.heavy[entry], make_int2(v, (int)c0), coh);
             for (uint32_t c = 0; c < nch; ++c) coh_st(&a.chunk_map[c0 + c], (int32_t)entry, coh);
         } else {
@@ -195,7 +194,7 @@ struct PairSink {
 };
 
 // The deepest label: one atomic per workgroup that found a vertex.  A
-// persistent launch (a.coherent) leaves it to workgroup 0, which learns
+// persistent launch leaves it to workgroup 0, which learns
 // after the level's barrier whether the next frontier is empty (every
 // vertex a top-down step claims is enqueued), so the grid's workgroups do
 // not all hit one counter at every level.
@@ -203,7 +202,7 @@ template <int VCAP>
 __device__ __forceinline__ void td_finish(const BfsArgs& a, int ph, TdSink<VCAP>& sink, unsigned long long* red) {
     lq_flush<false>(sink.q, sink.out, 1);
     if (sink.count_edges) wg_add_u64(sink.eacc, &a.ctr->level_edges[(ph + 1) & 3], red);
-    if (a.coherent) return;
+    if (a.persistent) return;
     if (__syncthreads_or(sink.found) && threadIdx.x == 0) atomic_max_u32(a.max_depth, (uint32_t)sink.depth_next);
 }
 
@@ -227,7 +226,7 @@ __device__ void td_phase(const BfsArgs& a, int ph, int32_t depth_next, uint32_t*
         const uint32_t i = base + threadIdx.x;
         int32_t u = -1, start = 0, deg = 0;
         if (threadIdx.x < per && i < n_in) {
-            u = coh_ld(&qin[i], a.coherent != 0);
+            u = coh_ld(&qin[i], a.coh_in != 0);
             if (valid_vertex(a, u)) {
                 start = a.offs[u];
                 deg = a.offs[u + 1] - start;
@@ -259,8 +258,8 @@ __device__ void ce_phase(const BfsArgs& a, int ph, int32_t depth_here, LevelShar
         const uint32_t i = base + threadIdx.x;
         int32_t v = -1, start = 0, deg = 0;
         if (threadIdx.x < per && i < n_in) {
-            v = coh_ld(&qk[i], a.coherent != 0);
-            const int32_t p = coh_ld(&qv[i], a.coherent != 0);
+            v = coh_ld(&qk[i], a.coh_in != 0);
+            const int32_t p = coh_ld(&qv[i], a.coh_in != 0);
             bool keep = valid_vertex(a, v);
             if (keep) keep = !visited_test(a.visited, v);
             if (keep) keep = !history_seen(s.hist, v);
@@ -296,8 +295,8 @@ __device__ void tp_contract_phase(const BfsArgs& a, int ph, int32_t depth_here, 
         bool keep = false;
         int32_t v = -1;
         if (i < n_in) {
-            v = coh_ld(&qk[i], a.coherent != 0);
-            const int32_t p = coh_ld(&qv[i], a.coherent != 0);
+            v = coh_ld(&qk[i], a.coh_in != 0);
+            const int32_t p = coh_ld(&qv[i], a.coh_in != 0);
             keep = valid_vertex(a, v);
             if (keep) keep = !visited_test(a.visited, v);
             if (keep) keep = !history_seen(s.hist, v);
@@ -388,7 +387,7 @@ __device__ void tp_expand_phase(const BfsArgs& a, int ph, LevelShared& s, uint32
         const uint32_t i = base + threadIdx.x;
         int32_t u = -1, start = 0, deg = 0;
         if (i < n_in) {
-            u = coh_ld(&qin[i], a.coherent != 0);
+            u = coh_ld(&qin[i], a.coh_in != 0);
             if (valid_vertex(a, u)) {
                 start = a.offs[u];
                 deg = a.offs[u + 1] - start;
@@ -412,8 +411,8 @@ __device__ void tp_filter_phase(const BfsArgs& a, int ph, LevelShared& s) {
         bool keep = false;
         int32_t v = -1, p = 0;
         if (i < n_in) {
-            v = coh_ld(&qk[i], a.coherent != 0);
-            p = coh_ld(&qv[i], a.coherent != 0);
+            v = coh_ld(&qk[i], a.coh_in != 0);
+            p = coh_ld(&qv[i], a.coh_in != 0);
             keep = valid_vertex(a, v);
             if (keep) keep = !visited_test(a.visited, v);
             if (keep) keep = !history_seen(s.hist, v);
@@ -858,12 +857,12 @@ __device__ void heavy_body(BfsArgs a, int ph, int32_t depth_next, int slot, type
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const uint32_t c = c0 + (uint32_t)r * stride;
-            entry[r] = c < C ? coh_ld(&a.chunk_map[c], a.coherent != 0) : -1;
+            entry[r] = c < C ? coh_ld(&a.chunk_map[c], a.coh_out != 0) : -1;
         }
         int2 eu[R];
 #pragma unroll
         for (int r = 0; r < R; ++r)
-            eu[r] = (uint32_t)entry[r] < H ? coh_ld2(&a.heavy[entry[r]], a.coherent != 0) : make_int2(-1, 0);
+            eu[r] = (uint32_t)entry[r] < H ? coh_ld2(&a.heavy[entry[r]], a.coh_out != 0) : make_int2(-1, 0);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             const uint32_t c = c0 + (uint32_t)r * stride;
@@ -909,9 +908,9 @@ __device__ void heavy_body(BfsArgs a, int ph, int32_t depth_next, int slot, type
                 int32_t u = -1, start = 0, len = 0;
                 if (i < nb) {
                     const uint32_t c = blockIdx.x + (b0 + i) * stride;
-                    const int32_t entry = coh_ld(&a.chunk_map[c], a.coherent != 0);
+                    const int32_t entry = coh_ld(&a.chunk_map[c], a.coh_out != 0);
                     if ((uint32_t)entry < H) {
-                        const int2 eu = coh_ld2(&a.heavy[entry], a.coherent != 0);
+                        const int2 eu = coh_ld2(&a.heavy[entry], a.coh_out != 0);
                         if (valid_vertex(a, eu.x) && (uint32_t)eu.y <= c) {
                             const int32_t s0 = a.offs[eu.x], deg = a.offs[eu.x + 1] - s0;
                             const long long off = (long long)(c - (uint32_t)eu.y) * HEAVY_CHUNK;
@@ -1010,9 +1009,21 @@ __device__ void fused_body(BfsArgs a, int ph, int32_t depth, int max_levels, uin
     unsigned long long t0 = 0, busy = 0;
     if (a.wg_stats && threadIdx.x == 0) t0 = wg_clock();
     shared_init(s);
-    const GridBarrier bar{a.bar_arrive, &a.ctr->bar_release, &a.ctr->bar_abort, (uint32_t)a.coherent};
+    GridBarrier bar{a.bar_arrive, &a.ctr->bar_release, &a.ctr->bar_abort, 0u};
     uint32_t epoch = barrier_base_epoch(bar);
     if (blockIdx.x == 0 && threadIdx.x == 0) a.ctr->saturated = 0;
+    // Barrier mode per phase: a phase whose input holds at most light_limit
+    // entries writes its output (queue entries, grid-tier entries) with
+    // agent-scope accesses and ends with a light barrier; a wider phase
+    // writes plainly, keeping its queue traffic in L2, and ends with a
+    // fenced barrier.  Each phase reads its input in the mode it was written
+    // in.  All workgroups decide alike: the lengths come with the barriers.
+    a.persistent = 1;
+    a.coh_in = 0;  // the first input was written by an earlier launch
+    auto set_mode = [&](uint32_t n) {
+        a.coh_out = n <= a.light_limit ? 1 : 0;
+        bar.light = (uint32_t)a.coh_out;
+    };
     // Barrier payload: the length of the frontier queue in counter slot
     // `slot` (clamped to 31 bits: a frontier never holds 2^31 entries, since
     // E < 2^31) with bit 31 set when the grid tier has lists deferred by
@@ -1043,6 +1054,7 @@ __device__ void fused_body(BfsArgs a, int ph, int32_t depth, int max_levels, uin
         int heavy_ph = ph;          // phase whose deferred lists the grid tier takes
         int32_t heavy_depth = depth;  // depth label of what it discovers (top-down)
         uint32_t word = 0;
+        set_mode(n_in);
         if (KIND == FK_TD) {
             td_phase(a, ph, depth + 1, nullptr, false, s.q, s.g, s.red, gridDim.x, a.grid_th, n_in);
             heavy_depth = depth + 1;
@@ -1059,6 +1071,8 @@ __device__ void fused_body(BfsArgs a, int ph, int32_t depth, int max_levels, uin
             // has it); the overflow flag ends the search.
             uint32_t n_mid = word & 0x7FFFFFFFu;
             if (n_mid > a.qcap[(ph + 1) & 1]) n_mid = 0;
+            a.coh_in = a.coh_out;
+            set_mode(n_mid);
             tp_expand_phase(a, ph + 1, s, n_mid);
             heavy_ph = ph + 1;
             ph += 2;
@@ -1068,14 +1082,16 @@ __device__ void fused_body(BfsArgs a, int ph, int32_t depth, int max_levels, uin
         if (!grid_sync(bar, ++epoch, payload_for(ph, grid_tier ? heavy_ph : -1), &s.ctl, word)) break;
         if (word & 0x80000000u) {
             // Lists this level deferred: the whole grid takes them in
-            // chunks, appending to the same next frontier.
+            // chunks, appending to the same next frontier (in the level's
+            // mode: its entries were written so, and so is its output).
             heavy_body<FusedHeavy<KIND>::kind>(a, heavy_ph, heavy_depth, 0, hs);
             if (!grid_sync(bar, ++epoch, payload_for(ph, -1), &s.ctl, word)) break;
         }
+        a.coh_in = a.coh_out;
         n_in = word & 0x7FFFFFFFu;
         // Top-down: the level claimed vertices (at `depth`) iff the next
         // frontier is not empty (td_finish leaves this to workgroup 0).
-        if (KIND == FK_TD && a.coherent && n_in != 0 && blockIdx.x == 0 && threadIdx.x == 0)
+        if (KIND == FK_TD && n_in != 0 && blockIdx.x == 0 && threadIdx.x == 0)
             atomic_max_u32(a.max_depth, (uint32_t)depth);
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) {

@@ -124,7 +124,7 @@ def pct(x, q):
     return float(np.percentile(np.asarray(x), q)) if len(x) else None
 
 
-def bench_lattice(bfs, b4, hip, info, width=200, sources=4, steps=8):
+def bench_lattice(bfs, b4, hip, info, width=200, sources=4, steps=8, only=None):
     """High-diameter workload: 200^3 6-neighbour lattice, ids permuted from
     seed 0, `sources` seeded sources; GTEPS per strategy (one launch per
     level, levels captured in hipGraphs where the strategy allows)."""
@@ -142,6 +142,8 @@ def bench_lattice(bfs, b4, hip, info, width=200, sources=4, steps=8):
                              (b4.TWO_PHASE, b4.MODE_FUSED, "two_phase_persistent"),
                              (b4.HYBRID, b4.MODE_DEFAULT, "hybrid"),
                              (b4.DIRECTION_OPTIMIZING, b4.MODE_DEFAULT, "direction_optimizing")):
+        if only and name not in only:
+            continue
         t = tuning_for(s_id, g, info["arch"])
         e_trav, v_vis, levels = [], [], []
         for s in srcs:
@@ -728,6 +730,8 @@ def main():
                     help="skip the graph-file read/build, primitives and per-stage microbenchmark workloads")
     ap.add_argument("--only", choices=["batched", "served", "lattice", "sort"],
                     help="run one secondary workload alone and print its result (profiling runs)")
+    ap.add_argument("--lattice-strategies", default=None,
+                    help="with --only lattice: comma-separated subset (e.g. expand_contract_persistent)")
     ap.add_argument("--probe", action="store_true",
                     help="counter-collection run: a 1 GiB copy plus --steps searches, no JSON line")
     args = ap.parse_args()
@@ -749,7 +753,8 @@ def main():
         elif args.only == "served":
             res = bench_served(b4, info)
         elif args.only == "lattice":
-            res = bench_lattice(bfs, b4, Hip(), info)
+            res = bench_lattice(bfs, b4, Hip(), info,
+                                only=args.lattice_strategies.split(",") if args.lattice_strategies else None)
         else:
             res = bench_sort(Hip())
         print("RESULT " + json.dumps(res), flush=True)
