@@ -339,6 +339,56 @@ def bench_served(b4, info, duration_s=1.5):
     return out
 
 
+def bench_cohorts(b4, info, ks=(1, 4, 16, 64), reps=3):
+    """Cost of one cohort of k queries on the served pack's RMAT-23 graph
+    (RMAT-23 + 512 RMAT graphs of scale 12-16): all k admitted at one level
+    boundary and run to completion by the batched engine, one level per
+    step; beside it the plain direction-optimizing enactor per query.  The
+    served analysis and its kernel trace rest on this leg."""
+    from back40computing_amd import serve
+    from back40computing_amd.batch import MultiGraph, MultiSourceBFS
+    from back40computing_amd.bfs import Graph
+    from back40computing_amd import autotune
+    big = Graph.rmat(23, 16, 0.57, 0.19, 0.19, 7)
+    pool_scales = [12 + int(serve._uniform(77, k) * 5) for k in range(512)]
+    pool, _ = serve.build_pool(pool_scales, seed0=50000)
+    mg = MultiGraph([big] + pool)
+    off = np.empty(big.nodes + 1, np.int32)
+    b4._lib.check(b4.lib().b40c_graph_copy_to_host(big.handle, b4._lib.ptr(off), None))
+    srcs = [int(s) for s in pick_sources_cached(off, 11, max(ks) * reps)]
+    eng = MultiSourceBFS(mg, 6 << 30)
+    out = {"workload": "rmat23_served_pack_cohorts", "cohorts": []}
+    for k in ks:
+        for rep in range(reps):
+            for s in srcs[rep * k:(rep + 1) * k]:
+                eng.submit(0, s)
+            t0 = time.perf_counter()
+            levels = 0
+            while True:
+                r, w = eng.step()
+                levels += 1
+                if r == 0 and w == 0:
+                    break
+            ms = (time.perf_counter() - t0) * 1e3
+            for q in eng.poll():
+                eng.release(q)
+            out["cohorts"].append({"k": k, "levels": levels, "ms": round(ms, 3), "ms_per_query": round(ms / k, 4)})
+            log("cohort k=%d: %.3f ms (%.4f ms per query, %d levels)" % (k, ms, ms / k, levels))
+    plain = serve.PlainServer([big], b4.DIRECTION_OPTIMIZING, tuning_for=lambda g: autotune.lookup(
+        info["arch"], b4.DIRECTION_OPTIMIZING, big.nodes, big.edges))
+    for s in srcs[:4]:
+        plain.run_one(0, s)
+    t0 = time.perf_counter()
+    for s in srcs[:32]:
+        plain.run_one(0, s)
+    out["plain_ms_per_query"] = round((time.perf_counter() - t0) * 1e3 / 32, 4)
+    log("cohorts: plain %.4f ms per query" % out["plain_ms_per_query"])
+    eng.close()
+    plain.close()
+    mg.close()
+    return out
+
+
 def _ingest_one(graph_io, path, fmt, off, cols, write_s=None, reps=2):
     """Best of `reps` reads of one file: host parse (b40c_edge_list_read)
     and device CSR build (b40c_graph_from_edge_list); the CSR must be the
@@ -728,7 +778,7 @@ def main():
     ap.add_argument("--no-lattice", action="store_true", help="skip the 200^3 lattice (high-diameter) workload")
     ap.add_argument("--no-ingest", action="store_true",
                     help="skip the graph-file read/build, primitives and per-stage microbenchmark workloads")
-    ap.add_argument("--only", choices=["batched", "served", "lattice", "sort"],
+    ap.add_argument("--only", choices=["batched", "served", "lattice", "sort", "cohorts"],
                     help="run one secondary workload alone and print its result (profiling runs)")
     ap.add_argument("--lattice-strategies", default=None,
                     help="with --only lattice: comma-separated subset (e.g. expand_contract_persistent)")
@@ -752,6 +802,8 @@ def main():
             res = bench_batched(b4, info)
         elif args.only == "served":
             res = bench_served(b4, info)
+        elif args.only == "cohorts":
+            res = bench_cohorts(b4, info)
         elif args.only == "lattice":
             res = bench_lattice(bfs, b4, Hip(), info,
                                 only=args.lattice_strategies.split(",") if args.lattice_strategies else None)
