@@ -170,6 +170,7 @@ SIGNATURES = {
     "b40c_msbfs_set_tuning": (_i32, [_vp, _i32, _i32, _i32]),
     "b40c_msbfs_set_admission": (_i32, [_vp, _i32]),
     "b40c_msbfs_set_levels_per_step": (_i32, [_vp, _i32]),
+    "b40c_msbfs_set_persistent": (_i32, [_vp, _i32]),
     "b40c_msbfs_submit": (_i32, [_vp, _i32, _i32, _p_i32]),
     "b40c_msbfs_step": (_i32, [_vp, _p_i32, _p_i32]),
     "b40c_msbfs_run": (_i32, [_vp]),

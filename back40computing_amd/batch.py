@@ -65,6 +65,11 @@ class MultiSourceBFS:
         levels with one synchronisation; 0 launches one level per step."""
         check(lib().b40c_msbfs_set_levels_per_step(self._h, int(levels)))
 
+    def set_persistent(self, on):
+        """Run each step's levels in one persistent launch (grid barriers
+        between the level's steps) instead of launches or a captured graph."""
+        check(lib().b40c_msbfs_set_persistent(self._h, 1 if on else 0))
+
     def submit(self, graph, src):
         q = ctypes.c_int32()
         check(lib().b40c_msbfs_submit(self._h, int(graph), int(src), ctypes.byref(q)))

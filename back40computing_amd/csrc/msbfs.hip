@@ -48,9 +48,11 @@ struct alignas(256) MsCounters {
     uint32_t any_bu[2];  // some slot is in bottom-up mode (by level parity)
     uint32_t overflow;
     uint32_t corrupt;
-    uint32_t pad[8];
+    uint32_t bar_abort;  // persistent form: a grid-barrier wait timed out
+    uint32_t pad[7];
     unsigned long long heavy_alloc[2];  // (entries << 32 | chunks) deferred to the grid tier, by parity
     unsigned long long bu_tiles[2];     // (graphs << 32 | tiles) listed for the pull, by parity
+    unsigned long long bar_release;     // persistent form: grid-barrier release word (epoch | payload)
 };
 
 struct MsArgs {
@@ -78,6 +80,7 @@ struct MsArgs {
     MsCounters* hctr;                  // device address of the pinned host mirror (overflow, corrupt)
     int32_t* hdone;                    // [Q] device address of pinned host words: levels of a finished query
     int32_t* dph;                      // captured level loops: phase of the replay's first level (else null)
+    uint32_t* bar_arrive;              // persistent form: grid-barrier arrival word per workgroup
     int2* heavy;                       // grid-tier entries (vertex, first chunk)
     int32_t* chunk_map;                // chunk -> entry
     uint32_t qcap;
@@ -132,6 +135,7 @@ constexpr int MS_SPARSE_DIV = 16;
 // (more: every tile of the pack).
 constexpr int MS_TILE_TABLE = 512;
 constexpr int MS_TILE = BFS_THREADS;  // vertices per pull tile
+constexpr int kMsMaxFusedGrid = 4096;  // grid-barrier arrival words of the persistent form
 
 struct MsShared {
     LdsKeyQueue q;
@@ -222,9 +226,7 @@ __device__ __forceinline__ void ms_defer_heavy(const MsArgs& a, int ph, int32_t 
 // Top-down push of level ph (lists below the grid tier's threshold).
 // Vertices that gain bits are appended once (the first OR into next[v]
 // enqueues) to the next frontier queue.
-__global__ __launch_bounds__(BFS_THREADS) void k_ms_push(MsArgs a, int ph) {
-    ph = ms_phase(a, ph);
-    __shared__ MsShared s;
+__device__ void ms_push_body(const MsArgs& a, int ph, MsShared& s) {
     uint32_t n_f = ld_relaxed(&a.ctr->qlen[ph & 3]);
     if (n_f > a.qcap) n_f = 0;
     // Clear the mask array of level ph - 1 (it becomes level ph + 2's next
@@ -327,9 +329,7 @@ __device__ __forceinline__ unsigned long long group_excl_or(unsigned long long x
     return sub == 0 ? 0ull : e;
 }
 
-__global__ __launch_bounds__(BFS_THREADS) void k_ms_pull(MsArgs a, int ph) {
-    ph = ms_phase(a, ph);
-    __shared__ MsPullShared s;
+__device__ void ms_pull_body(const MsArgs& a, int ph, MsPullShared& s) {
     uint32_t n_f = ld_relaxed(&a.ctr->qlen[ph & 3]);
     if (n_f > a.qcap) n_f = 0;
     if (n_f == 0 || !ld_relaxed(&a.ctr->any_bu[ph & 1])) return;
@@ -451,9 +451,7 @@ __global__ __launch_bounds__(BFS_THREADS) void k_ms_pull(MsArgs a, int ph) {
 // Grid tier of level ph's push: workgroups take chunks grid-stride, one edge
 // per thread.  Each chunk is checked against the entry it names, so a stale
 // map word never indexes outside a list.
-__global__ __launch_bounds__(BFS_THREADS) void k_ms_heavy(MsArgs a, int ph) {
-    ph = ms_phase(a, ph);
-    __shared__ MsShared s;
+__device__ void ms_heavy_body(const MsArgs& a, int ph, MsShared& s) {
     const unsigned long long alloc =
         __hip_atomic_load(&a.ctr->heavy_alloc[ph & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const uint32_t H = min((uint32_t)(alloc >> 32), a.heavy_cap);
@@ -493,8 +491,7 @@ __global__ __launch_bounds__(BFS_THREADS) void k_ms_heavy(MsArgs a, int ph) {
 // mostly share a graph) and, per chunk of 64 entries and per graph present in
 // it, transposes the lanes' bit masks so that lane b holds the set of lanes
 // carrying bit b.  Running totals are kept while the graph stays the same.
-__global__ __launch_bounds__(BFS_THREADS) void k_ms_count(MsArgs a, int ph) {
-    ph = ms_phase(a, ph);
+__device__ void ms_count_body(const MsArgs& a, int ph) {
     uint32_t n_q = ld_relaxed(&a.ctr->qlen[(ph + 1) & 3]);
     // Frontier bitmap of level ph + 1: set entry by entry from the queue while
     // it is short (the array is all zero: its bits were those of level ph - 1,
@@ -588,64 +585,115 @@ __global__ __launch_bounds__(BFS_THREADS) void k_ms_count(MsArgs a, int ph) {
 // same captured replay, skip it) and the query's level count is written to
 // the pinned host word hdone[q]; the host frees the slot and its labels at
 // its next level boundary (k_ms_release clears the slot's visited bits).
-__global__ __launch_bounds__(BFS_THREADS) void k_ms_direction(MsArgs a, int ph) {
-    ph = ms_phase(a, ph);
-    const int32_t g = (int32_t)(blockIdx.x * BFS_WAVES + threadIdx.x / kWave);
+// One graph per wave, waves grid-strided over the graphs.
+__device__ void ms_direction_body(const MsArgs& a, int ph) {
     if (blockIdx.x == 0 && threadIdx.x == 0) st_relaxed(&a.ctr->any_bu[ph & 1], 0u);
-    if (g >= a.G) return;
     const int b = lane_id();
-    const size_t k = (size_t)g * 64 + b;
-    const uint32_t n = a.cnt_n[k];
-    const unsigned long long m = a.cnt_m[k];
-    a.cnt_n[k] = 0u;
-    a.cnt_m[k] = 0ull;
-    const bool occupied = (a.active[g] >> b) & 1ull;
-    const bool was_bu = (a.bu[g] >> b) & 1ull;
-    const long long Vg = a.vbase[g + 1] - a.vbase[g];
-    const long long Eg = a.ebase[g + 1] - a.ebase[g];
-    bool to_bu = false;
-    if (occupied && n) {
-        const unsigned long long seen = a.m_seen[k] + m;
-        a.m_seen[k] = seen;
-        if (!was_bu) {
-            long long m_u = Eg - (long long)seen;
-            if (m_u < 0) m_u = 0;
-            to_bu = (long long)m * (long long)a.alpha > m_u;
-        } else {
-            to_bu = !((long long)n * (long long)a.beta < Vg && n < a.nf_prev[k]);
+    for (int32_t g = (int32_t)(blockIdx.x * BFS_WAVES + threadIdx.x / kWave); g < a.G;
+         g += (int32_t)(gridDim.x * BFS_WAVES)) {
+        const size_t k = (size_t)g * 64 + b;
+        const uint32_t n = a.cnt_n[k];
+        const unsigned long long m = a.cnt_m[k];
+        a.cnt_n[k] = 0u;
+        a.cnt_m[k] = 0ull;
+        const bool occupied = (a.active[g] >> b) & 1ull;
+        const bool was_bu = (a.bu[g] >> b) & 1ull;
+        const long long Vg = a.vbase[g + 1] - a.vbase[g];
+        const long long Eg = a.ebase[g + 1] - a.ebase[g];
+        bool to_bu = false;
+        if (occupied && n) {
+            const unsigned long long seen = a.m_seen[k] + m;
+            a.m_seen[k] = seen;
+            if (!was_bu) {
+                long long m_u = Eg - (long long)seen;
+                if (m_u < 0) m_u = 0;
+                to_bu = (long long)m * (long long)a.alpha > m_u;
+            } else {
+                to_bu = !((long long)n * (long long)a.beta < Vg && n < a.nf_prev[k]);
+            }
+        }
+        a.nf_prev[k] = n;
+        const unsigned long long bum = __ballot(to_bu);
+        const bool finished = occupied && n == 0u;
+        const unsigned long long done = __ballot(finished);
+        if (finished) {
+            const int32_t q = a.bitq[k];
+            if (q >= 0) a.hdone[q] = ph + 1 - a.qstart[q];
+        }
+        if (b == 0) {
+            a.bu[g] = bum;
+            if (done) atomic_and_u64(&a.active[g], ~done);
+            if (bum) {
+                // List the graph for the next level's pull: one 64-bit atomic
+                // gives both its table entry and its first tile, so the table
+                // is in tile order.
+                st_relaxed(&a.ctr->any_bu[(ph + 1) & 1], 1u);
+                const uint32_t nt = (uint32_t)((Vg + MS_TILE - 1) / MS_TILE);
+                const unsigned long long t = __hip_atomic_fetch_add(
+                    &a.ctr->bu_tiles[(ph + 1) & 1], (1ull << 32) | (unsigned long long)nt, __ATOMIC_RELAXED,
+                    __HIP_MEMORY_SCOPE_AGENT);
+                const uint32_t e = (uint32_t)(t >> 32);
+                if (e < (uint32_t)a.G) a.bu_list[(size_t)((ph + 1) & 1) * a.G + e] = make_int2(g, (int)(uint32_t)t);
+            }
         }
     }
-    a.nf_prev[k] = n;
-    const unsigned long long bum = __ballot(to_bu);
-    const bool finished = occupied && n == 0u;
-    const unsigned long long done = __ballot(finished);
-    if (finished) {
-        const int32_t q = a.bitq[k];
-        if (q >= 0) a.hdone[q] = ph + 1 - a.qstart[q];
-    }
-    if (b == 0) {
-        a.bu[g] = bum;
-        if (done) atomic_and_u64(&a.active[g], ~done);
-        if (bum) {
-            // List the graph for the next level's pull: one 64-bit atomic
-            // gives both its table entry and its first tile, so the table
-            // is in tile order.
-            st_relaxed(&a.ctr->any_bu[(ph + 1) & 1], 1u);
-            const uint32_t nt = (uint32_t)((Vg + MS_TILE - 1) / MS_TILE);
-            const unsigned long long t = __hip_atomic_fetch_add(
-                &a.ctr->bu_tiles[(ph + 1) & 1], (1ull << 32) | (unsigned long long)nt, __ATOMIC_RELAXED,
-                __HIP_MEMORY_SCOPE_AGENT);
-            const uint32_t e = (uint32_t)(t >> 32);
-            if (e < (uint32_t)a.G) a.bu_list[(size_t)((ph + 1) & 1) * a.G + e] = make_int2(g, (int)(uint32_t)t);
-        }
-    }
-    // The level's last launch: publish the error flags beside the finished
+    // The level's last step: publish the error flags beside the finished
     // queries' words (the host reads both after one stream synchronisation,
     // no copies).
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         a.hctr->overflow = ld_relaxed(&a.ctr->overflow);
         a.hctr->corrupt = ld_relaxed(&a.ctr->corrupt);
+        a.hctr->bar_abort = ld_relaxed(&a.ctr->bar_abort);
     }
+}
+
+// The level kernels: each wraps its body (launched one after another, or
+// replayed from a captured graph with the phase read from device memory).
+__global__ __launch_bounds__(BFS_THREADS) void k_ms_push(MsArgs a, int ph) {
+    __shared__ MsShared s;
+    ms_push_body(a, ms_phase(a, ph), s);
+}
+__global__ __launch_bounds__(BFS_THREADS) void k_ms_heavy(MsArgs a, int ph) {
+    __shared__ MsShared s;
+    ms_heavy_body(a, ms_phase(a, ph), s);
+}
+__global__ __launch_bounds__(BFS_THREADS) void k_ms_pull(MsArgs a, int ph) {
+    __shared__ MsPullShared s;
+    ms_pull_body(a, ms_phase(a, ph), s);
+}
+__global__ __launch_bounds__(BFS_THREADS) void k_ms_count(MsArgs a, int ph) { ms_count_body(a, ms_phase(a, ph)); }
+__global__ __launch_bounds__(BFS_THREADS) void k_ms_direction(MsArgs a, int ph) {
+    ms_direction_body(a, ms_phase(a, ph));
+}
+
+// Persistent form: `levels` whole levels in one launch of a co-resident grid,
+// the five steps of a level separated by grid barriers (fenced mode: the
+// steps pass masks, queues and counters through plain accesses).  Queries
+// that finish retire on the device as in the other forms; a level after the
+// last query finished costs only its barriers.
+union MsFusedShared {
+    MsShared push;
+    MsPullShared pull;
+};
+__global__ __launch_bounds__(BFS_THREADS) void k_ms_fused(MsArgs a, int ph0, int levels) {
+    __shared__ MsFusedShared s;
+    __shared__ uint32_t s_word;
+    const GridBarrier bar{a.bar_arrive, &a.ctr->bar_release, &a.ctr->bar_abort, 0u};
+    uint32_t epoch = barrier_base_epoch(bar);
+    for (int l = 0; l < levels; ++l) {
+        const int ph = ph0 + l;
+        ms_push_body(a, ph, s.push);
+        if (!grid_sync(bar, ++epoch, &s_word)) break;
+        ms_heavy_body(a, ph, s.push);
+        if (!grid_sync(bar, ++epoch, &s_word)) break;
+        ms_pull_body(a, ph, s.pull);
+        if (!grid_sync(bar, ++epoch, &s_word)) break;
+        ms_count_body(a, ph);
+        if (!grid_sync(bar, ++epoch, &s_word)) break;
+        ms_direction_body(a, ph);
+        if (!grid_sync(bar, ++epoch, &s_word)) break;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.hctr->bar_abort = ld_relaxed(&a.ctr->bar_abort);
 }
 
 // Last node of a captured level loop: moves the replay's first phase on by
@@ -853,6 +901,9 @@ struct b40c_msbfs {
     // Captured level loop (levels_per_replay > 0): the five level launches of
     // `levels_per_replay` levels and k_ms_advance, replayed as one graph.
     int32_t levels_per_replay = 0;
+    bool persistent = false;             // levels_per_replay levels per step in one k_ms_fused launch
+    b40c::DeviceBuffer bar_arrive;       // its grid-barrier arrival words
+    bool bar_dirty = false;              // a barrier wait timed out: clear the barrier state first
     hipGraph_t loop_graph = nullptr;
     hipGraphExec_t loop_exec = nullptr;
     int32_t loop_levels = 0, loop_grid = 0, loop_alpha = 0, loop_beta = 0;
@@ -897,6 +948,7 @@ MsArgs ms_args(b40c_msbfs* m) {
     a.hctr = m->h_ctr_dev;
     a.hdone = m->h_done_dev;
     a.dph = nullptr;
+    a.bar_arrive = m->bar_arrive.as<uint32_t>();
     a.heavy = m->heavy.as<int2>();
     a.chunk_map = m->chunk_map.as<int32_t>();
     a.heavy_cap = m->heavy_cap;
@@ -1043,6 +1095,23 @@ void ms_launch_level(b40c_msbfs* m, const MsArgs& a, int ph) {
     hipLaunchKernelGGL(k_ms_count, dim3(m->grid), dim3(BFS_THREADS), 0, m->stream, a, ph);
     hipLaunchKernelGGL(k_ms_direction, dim3((m->G + BFS_WAVES - 1) / BFS_WAVES), dim3(BFS_THREADS), 0, m->stream, a,
                        ph);
+}
+
+// One k_ms_fused launch of `levels` levels: the largest co-resident grid
+// (occupancy API) capped at the level kernels' push grid, a plain launch (no
+// cooperative-launch path).
+b40c_status ms_launch_fused(b40c_msbfs* m, int levels) {
+    DeviceInfo info;
+    B40C_TRY(device_info(info));
+    int per_cu = 0;
+    B40C_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_ms_fused, BFS_THREADS, 0));
+    if (per_cu < 1) return fail(B40C_ERR_UNSUPPORTED, "batched persistent kernel cannot be resident");
+    const int grid = std::min({per_cu * info.num_cus, m->grid, kMsMaxFusedGrid});
+    MsArgs a = ms_args(m);
+    int ph0 = m->ph;
+    void* args[] = {&a, &ph0, &levels};
+    B40C_HIP(hipLaunchKernel((const void*)k_ms_fused, dim3((unsigned)grid), dim3(BFS_THREADS), args, 0, m->stream));
+    return B40C_OK;
 }
 
 // Captures levels_per_replay levels plus k_ms_advance into one hipGraph
@@ -1196,6 +1265,7 @@ b40c_status b40c_msbfs_create(b40c_multigraph* mg, int64_t label_budget_bytes, i
     if (s == B40C_OK) s = m->active.alloc((size_t)m->G * 8);
     if (s == B40C_OK) s = m->bu.alloc((size_t)m->G * 8);
     if (s == B40C_OK) s = m->dph.alloc(sizeof(uint32_t));
+    if (s == B40C_OK) s = m->bar_arrive.alloc((size_t)kMsMaxFusedGrid * sizeof(uint32_t));
     if (s == B40C_OK) s = m->cnt_n.alloc((size_t)m->G * 64 * 4);
     if (s == B40C_OK) s = m->cnt_m.alloc((size_t)m->G * 64 * 8);
     if (s == B40C_OK) s = m->nf_prev.alloc((size_t)m->G * 64 * 4);
@@ -1224,6 +1294,7 @@ b40c_status b40c_msbfs_create(b40c_multigraph* mg, int64_t label_budget_bytes, i
     B40C_HIP(hipMemsetAsync(m->visited.ptr, 0, m->visited.bytes, m->stream));
     for (int i = 0; i < 3; ++i) B40C_HIP(hipMemsetAsync(m->front[i].ptr, 0, m->front[i].bytes, m->stream));
     B40C_HIP(hipMemsetAsync(m->dph.ptr, 0, m->dph.bytes, m->stream));
+    B40C_HIP(hipMemsetAsync(m->bar_arrive.ptr, 0, m->bar_arrive.bytes, m->stream));
     B40C_HIP(hipMemsetAsync(m->bu.ptr, 0, m->bu.bytes, m->stream));
     B40C_HIP(hipMemsetAsync(m->fbm.ptr, 0, m->fbm.bytes, m->stream));
     B40C_HIP(hipMemsetAsync(m->cnt_n.ptr, 0, m->cnt_n.bytes, m->stream));
@@ -1287,8 +1358,20 @@ b40c_status b40c_msbfs_step(b40c_msbfs* m, int32_t* running, int32_t* waiting) {
     if (!m) return fail(B40C_ERR_INVALID_ARG, "b40c_msbfs_step: null");
     B40C_TRY(ms_admit(m));
     if (!m->running.empty()) {
+        if (m->bar_dirty) {
+            // A persistent launch's barrier wait timed out: clear the barrier
+            // state (every form publishes its abort flag).
+            B40C_HIP(hipMemsetAsync(m->bar_arrive.ptr, 0, m->bar_arrive.bytes, m->stream));
+            B40C_HIP(hipMemsetAsync(&m->counters.as<MsCounters>()->bar_release, 0, sizeof(unsigned long long),
+                                    m->stream));
+            B40C_HIP(hipMemsetAsync(&m->counters.as<MsCounters>()->bar_abort, 0, sizeof(uint32_t), m->stream));
+            m->bar_dirty = false;
+        }
         int levels = 1;
-        if (m->levels_per_replay > 0) {
+        if (m->persistent) {
+            levels = std::max(1, m->levels_per_replay);
+            B40C_TRY(ms_launch_fused(m, levels));
+        } else if (m->levels_per_replay > 0) {
             levels = m->levels_per_replay;
             MsArgs a = ms_args(m);
             a.dph = m->dph.as<int32_t>();
@@ -1308,6 +1391,10 @@ b40c_status b40c_msbfs_step(b40c_msbfs* m, int32_t* running, int32_t* waiting) {
         // k_ms_direction has written h_ctr's flags and the finished
         // queries' done words.
         B40C_HIP(hipStreamSynchronize(m->stream));
+        if (m->h_ctr->bar_abort) {
+            m->bar_dirty = true;
+            return fail(B40C_ERR_HIP, "batched traversal: a grid-barrier wait timed out (workgroups not co-resident)");
+        }
         if (m->h_ctr->overflow || m->h_ctr->corrupt)
             return fail(B40C_ERR_HIP, "batched traversal: frontier overflow or corrupt frontier");
         B40C_TRY(ms_retire(m));
@@ -1322,6 +1409,12 @@ b40c_status b40c_msbfs_set_levels_per_step(b40c_msbfs* m, int32_t levels) {
     if (!m || levels < 0 || levels > 64)
         return fail(B40C_ERR_INVALID_ARG, "b40c_msbfs_set_levels_per_step: bad arguments");
     m->levels_per_replay = levels;
+    return B40C_OK;
+}
+
+b40c_status b40c_msbfs_set_persistent(b40c_msbfs* m, int32_t on) {
+    if (!m || (on != 0 && on != 1)) return fail(B40C_ERR_INVALID_ARG, "b40c_msbfs_set_persistent: bad arguments");
+    m->persistent = on != 0;
     return B40C_OK;
 }
 
@@ -1379,6 +1472,7 @@ void b40c_msbfs_destroy(b40c_msbfs* m) {
     m->bu.release();
     m->release_loop();
     m->dph.release();
+    m->bar_arrive.release();
     m->cnt_n.release();
     m->cnt_m.release();
     m->nf_prev.release();

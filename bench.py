@@ -191,9 +191,10 @@ def bench_batched(b4, info):
     plain_s = time.perf_counter() - t0
     budget = int(sum(graphs[g].nodes for g, _ in queries) * 8 * 1.05) + (1 << 20)
 
-    def run_batched(levels_per_step):
+    def run_batched(levels_per_step, persistent=False):
         eng = MultiSourceBFS(mg, budget, len(queries) + 8)
         eng.set_levels_per_step(levels_per_step)
+        eng.set_persistent(persistent)
         ids, done_ms = {}, {}
         t0 = time.perf_counter()
         for g, s in queries:
@@ -209,13 +210,15 @@ def bench_batched(b4, info):
                 break
         return eng, ids, done_ms, steps, time.perf_counter() - t0
 
-    # Two launch forms of the same engine: five launches and a host
-    # synchronisation per level, and a captured hipGraph replaying
-    # BATCH_LEVELS_PER_STEP levels per synchronisation.
+    # Three launch forms of the same engine: five launches and a host
+    # synchronisation per level; a captured hipGraph replaying
+    # BATCH_LEVELS_PER_STEP levels per synchronisation; one persistent launch
+    # of BATCH_LEVELS_PER_STEP levels (grid barriers between the steps).
     forms = {}
-    for name, lps in (("launch", 0), ("graph", BATCH_LEVELS_PER_STEP)):
-        run_batched(lps)[0].close()  # warm-up (capture, first-touch of the label arena)
-        eng, ids, done_ms, steps, batch_s = run_batched(lps)
+    for name, lps, pers in (("launch", 0, False), ("graph", BATCH_LEVELS_PER_STEP, False),
+                            ("persistent", BATCH_LEVELS_PER_STEP, True)):
+        run_batched(lps, pers)[0].close()  # warm-up (capture, first-touch of the label arena)
+        eng, ids, done_ms, steps, batch_s = run_batched(lps, pers)
         mismatches = 0
         for q, (g, s) in ids.items():
             d_b, _, _ = eng.result(q, g, want_pred=False)
@@ -223,7 +226,7 @@ def bench_batched(b4, info):
             mismatches += int(not np.array_equal(d_b, d_p))
         eng.close()
         lat = list(done_ms.values())
-        forms[name] = {"levels_per_step": lps, "steps": steps, "batched_qps": round(len(queries) / batch_s, 1),
+        forms[name] = {"levels_per_step": lps, "persistent": pers, "steps": steps, "batched_qps": round(len(queries) / batch_s, 1),
                        "speedup": round(plain_s / batch_s, 2), "batched_ms_p50": round(pct(lat, 50), 3),
                        "batched_ms_p99": round(pct(lat, 99), 3), "label_mismatches": mismatches}
         log("batched %s: %s" % (name, forms[name]))
@@ -235,6 +238,7 @@ def bench_batched(b4, info):
     mismatches = sum(f["label_mismatches"] for f in forms.values())
     res = {
         "queries": len(queries), "graphs": len(graphs), "levels_per_step": best["levels_per_step"],
+        "persistent": best["persistent"],
         "plain_qps": round(len(queries) / plain_s, 1), "batched_qps": best["batched_qps"],
         "speedup": best["speedup"],
         "plain_ms_p50": round(pct(plain_ms, 50), 4), "plain_ms_p99": round(pct(plain_ms, 99), 4),
@@ -244,8 +248,8 @@ def bench_batched(b4, info):
         "labels_identical": mismatches == 0, "label_mismatches": mismatches,
         "note": "batched latency = batch start -> query finished (all 1,024 submitted at once); plain_ms = one "
                 "query alone (Reset + EnactSearch, direction-optimizing); plain_fifo_ms = the same all-at-once "
-                "arrival served one query after another by the plain enactor; headline fields from the faster "
-                "launch form, labels of both forms compared with the plain enactor's for every query",
+                "arrival served one query after another by the plain enactor; headline fields from the fastest "
+                "launch form, labels of every form compared with the plain enactor's for every query",
     }
     plain.close()
     mg.close()
@@ -383,6 +387,43 @@ def bench_cohorts(b4, info, ks=(1, 4, 16, 64), reps=3):
         plain.run_one(0, s)
     out["plain_ms_per_query"] = round((time.perf_counter() - t0) * 1e3 / 32, 4)
     log("cohorts: plain %.4f ms per query" % out["plain_ms_per_query"])
+    # Plain searches side by side: N host threads, each with its own
+    # problem, enactor and HIP stream (the library's calls release the GIL).
+    import threading
+    from back40computing_amd import bfs as bfsmod
+    t = autotune.lookup(info["arch"], b4.DIRECTION_OPTIMIZING, big.nodes, big.edges)
+    out["plain_concurrent"] = []
+    for n in (1, 2, 4, 8):
+        slots = []
+        for k in range(n):
+            p = bfsmod.CsrProblem()
+            b4._lib.check(p.FromGraph(big))
+            en = bfsmod.ENACTORS[b4.DIRECTION_OPTIMIZING]()
+            if t:
+                en.set_tuning(**t)
+            b4._lib.check(p.Reset(en.GetFrontierType(), 1.3))
+            b4._lib.check(b4.lib().b40c_enact_search(en.handle, p.handle, srcs[k], 0, 0))
+            slots.append((p, en))
+        per = 24
+
+        def run(k):
+            p, en = slots[k]
+            for i in range(per):
+                b4._lib.check(p.Reset(en.GetFrontierType(), 1.3))
+                b4._lib.check(b4.lib().b40c_enact_search(en.handle, p.handle, srcs[(k * per + i) % len(srcs)], 0, 0))
+
+        t0 = time.perf_counter()
+        th = [threading.Thread(target=run, args=(k,)) for k in range(n)]
+        for x in th:
+            x.start()
+        for x in th:
+            x.join()
+        rate = n * per / (time.perf_counter() - t0)
+        out["plain_concurrent"].append({"streams": n, "searches_per_s": round(rate, 1)})
+        log("cohorts: %d plain searches side by side: %.0f searches/s" % (n, rate))
+        for p, en in slots:
+            p.close()
+            en.close()
     eng.close()
     plain.close()
     mg.close()

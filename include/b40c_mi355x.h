@@ -306,6 +306,15 @@ b40c_status b40c_msbfs_set_admission(b40c_msbfs* m, int32_t cohort_gap);
  * finish inside the replay are retired on the device and reported at its
  * end.  0 (default): each step launches one level's kernels directly. */
 b40c_status b40c_msbfs_set_levels_per_step(b40c_msbfs* m, int32_t levels);
+/* Persistent form (north-star batched form; no counterpart in the
+ * reference): on = 1 runs each step's levels (levels_per_step, at least 1)
+ * in one launch of a co-resident grid, the five steps of every level
+ * separated by software grid barriers; 0 (default) uses launches or the
+ * captured graph.  The grid is the occupancy API's co-resident maximum
+ * capped by the tuning's grid; a barrier wait that times out (workgroups not
+ * co-resident) fails the step with B40C_ERR_HIP, after which the queries in
+ * flight hold partial labels (destroy the engine). */
+b40c_status b40c_msbfs_set_persistent(b40c_msbfs* m, int32_t on);
 /* Queues a query (source `src` local to graph `graph`); returns its id.
  * B40C_ERR_ADMISSION if its labels can never fit the budget. */
 b40c_status b40c_msbfs_submit(b40c_msbfs* m, int32_t graph, int32_t src, int32_t* query_id);

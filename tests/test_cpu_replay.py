@@ -15,6 +15,7 @@ entered the vertex frontier and were used as addresses (a sanitizer SEGV with
 0x3C fill, a heap overflow at offsets[-1] with 0xFF fill, wrong labels with
 zero fill).
 """
+import fcntl
 import os
 import shutil
 import subprocess
@@ -37,7 +38,10 @@ DEFAULT, FUSED, ITERATIVE = 0, 1, 2
 def replay_exe():
     if shutil.which("g++") is None:
         pytest.skip("g++ is not available")
-    subprocess.run(["make", "-s", "-C", REPLAY_DIR], check=True)
+    # One build at a time (parallel test workers share the output file).
+    with open(os.path.join(REPLAY_DIR, ".build.lock"), "w") as lk:
+        fcntl.flock(lk, fcntl.LOCK_EX)
+        subprocess.run(["make", "-s", "-C", REPLAY_DIR], check=True)
     return EXE
 
 

@@ -57,7 +57,18 @@ def sources(orc, host, g, count, seed):
 # Levels per step: 0 launches each level's kernels, k >= 1 replays a
 # captured hipGraph of k levels (queries finishing mid-replay retire on the
 # device).
-LEVELS_PER_STEP = [pytest.param(0, id="launch"), pytest.param(1, id="graph1"), pytest.param(4, id="graph4")]
+LEVELS_PER_STEP = [pytest.param(0, id="launch"), pytest.param(1, id="graph1"), pytest.param(4, id="graph4"),
+                   pytest.param(-1, id="persistent1"), pytest.param(-3, id="persistent3")]
+
+
+def set_form(eng, lps):
+    """lps >= 0: launches (0) or a captured graph of lps levels; lps < 0: the
+    persistent form with -lps levels per step."""
+    if lps < 0:
+        eng.set_persistent(True)
+        eng.set_levels_per_step(-lps)
+    else:
+        eng.set_levels_per_step(lps)
 
 
 @pytest.mark.parametrize("lps", LEVELS_PER_STEP)
@@ -66,21 +77,22 @@ def test_batch_all_graphs(mods, orc, pool, lps):
     mg, host = pool
     queries = [(g, s) for g in range(mg.ngraphs) for s in sources(orc, host, g, 4, 10 + g)]
     eng = batch.MultiSourceBFS(mg, label_budget_bytes=1 << 28)
-    eng.set_levels_per_step(lps)
+    set_form(eng, lps)
     print("batch all graphs levels_per_step=%d" % lps, flush=True)
     res = run_all(eng, mg, queries)
     for q, ((g, s), (depth, pred, sd)) in res.items():
         check_query(orc, host, g, s, depth, pred, sd)
 
 
-@pytest.mark.parametrize("lps", [pytest.param(0, id="launch"), pytest.param(3, id="graph3")])
+@pytest.mark.parametrize("lps", [pytest.param(0, id="launch"), pytest.param(3, id="graph3"),
+                                 pytest.param(-2, id="persistent2")])
 def test_alone_equals_in_mix(mods, orc, pool, lps):
     """The mix runs in the given launch form, each query alone in the other."""
     bfs, batch = mods
     mg, host = pool
     mix = [(g, s) for g in range(mg.ngraphs) for s in sources(orc, host, g, 6, 20 + g)]
     eng = batch.MultiSourceBFS(mg, label_budget_bytes=1 << 28)
-    eng.set_levels_per_step(lps)
+    set_form(eng, lps)
     in_mix = {(g, s): r for q, ((g, s), r) in run_all(eng, mg, mix).items()}
     for g, s in mix[::5]:
         alone_eng = batch.MultiSourceBFS(mg, label_budget_bytes=1 << 28)
@@ -101,7 +113,7 @@ def test_continuous_batching_and_slot_reuse(mods, orc, pool, lps):
     g_big = 2
     budget = 8 * (3 * mg.nodes(g_big))  # room for about three queries on the largest graph
     eng = batch.MultiSourceBFS(mg, label_budget_bytes=budget)
-    eng.set_levels_per_step(lps)
+    set_form(eng, lps)
     waves = [[(g_big, s) for s in sources(orc, host, g_big, 5, 31)],
              [(1, s) for s in sources(orc, host, 1, 70, 32)],
              [(3, s) for s in sources(orc, host, 3, 6, 33)] + [(5, s) for s in sources(orc, host, 5, 3, 34)]]
@@ -189,7 +201,8 @@ def test_admission_policy_does_not_change_labels(mods, orc, pool):
         batch.MultiSourceBFS(mg, label_budget_bytes=1 << 20).set_admission(0)
 
 
-@pytest.mark.parametrize("gap,lps", [(1, 0), (6, 0), (1, 2)], ids=["gap1-launch", "gap6-launch", "gap1-graph2"])
+@pytest.mark.parametrize("gap,lps", [(1, 0), (6, 0), (1, 2), (1, -1)],
+                         ids=["gap1-launch", "gap6-launch", "gap1-graph2", "gap1-persistent1"])
 def test_staggered_sessions_with_idle_gaps(mods, orc, gap, lps):
     """Long continuous run: one query arrives at every level boundary, the
     engine drains to idle between sessions and restarts, slots are reused.
@@ -206,7 +219,7 @@ def test_staggered_sessions_with_idle_gaps(mods, orc, gap, lps):
     mg = batch.MultiGraph(graphs)
     eng = batch.MultiSourceBFS(mg, label_budget_bytes=1 << 28)
     eng.set_admission(gap)
-    eng.set_levels_per_step(lps)
+    set_form(eng, lps)
     rng = np.random.default_rng(70 + gap)
     ids, out = {}, []
     submitted = 0
@@ -233,7 +246,8 @@ def test_staggered_sessions_with_idle_gaps(mods, orc, gap, lps):
     mg.close()
 
 
-@pytest.mark.parametrize("lps", [pytest.param(0, id="launch"), pytest.param(2, id="graph2")])
+@pytest.mark.parametrize("lps", [pytest.param(0, id="launch"), pytest.param(2, id="graph2"),
+                                 pytest.param(-2, id="persistent2")])
 def test_query_ids_recycled_and_pack_released_first(mods, orc, lps):
     """max_queries bounds the queries submitted and not yet collected: ids
     come back after their results are taken, so a long-running engine keeps
@@ -245,7 +259,7 @@ def test_query_ids_recycled_and_pack_released_first(mods, orc, lps):
     host = [g.to_host() for g in graphs]
     mg = batch.MultiGraph(graphs)
     eng = batch.MultiSourceBFS(mg, label_budget_bytes=1 << 24, max_queries=8)
-    eng.set_levels_per_step(lps)
+    set_form(eng, lps)
     nodes = [mg.nodes(0), mg.nodes(1)]
     mg.close()  # the engine keeps the pack alive
     seen = set()
