@@ -134,7 +134,7 @@ constexpr int MS_SPARSE_DIV = 16;
 // The pull walks the tiles of at most this many graphs through an LDS table
 // (more: every tile of the pack).
 constexpr int MS_TILE_TABLE = 512;
-constexpr int MS_TILE = BFS_THREADS;  // vertices per pull tile
+constexpr int MS_TILE = 64;  // vertices per pull tile
 constexpr int kMsMaxFusedGrid = 4096;  // grid-barrier arrival words of the persistent form
 
 struct MsShared {
@@ -289,44 +289,102 @@ __device__ void ms_push_body(const MsArgs& a, int ph, MsShared& s) {
 // both use atomics there.  A neighbour's 64-bit mask is loaded only when the
 // frontier bitmap (one bit per vertex: "carries any bit this level", built
 // by k_ms_count, small enough to stay in L2) says it is in some frontier.
-// Workgroups take tiles of BFS_THREADS consecutive vertices:
-//  * stage 1: each lane checks the first MS_FIRST neighbours of its vertex;
+//
+// Each wave works alone on tiles of MS_TILE = 64 consecutive vertices of one
+// graph (lane l = vertex base + l), loading the next tile's visited masks
+// and row offsets one tile ahead, and skipping a tile with nothing wanted:
+//  * stage 1: each lane checks the first MS_FIRST neighbours of its vertex
+//    (loads issued together);
 //  * stage 2: vertices still missing wanted bits are scanned by MS_GROUP-lane
-//    groups, MS_GROUP neighbours per step; each new bit is credited to the
-//    first neighbour (in list order) that carries it.
+//    groups, MS_STRIDE neighbours per lane per step (loads issued together).
+// Each new bit is credited to the first neighbour (in list order) that
+// carries it, as a credit event (neighbour, bits): stage 1's stay in the
+// vertex's lane, stage 2's go to LDS (MS_EVENTS per vertex; further ones
+// write their labels at once).  The tile's labels are then written one query
+// (bit) at a time, lane l storing vertex base + l, so a wave writes 256
+// contiguous bytes per label array instead of scattering one word per query
+// per vertex over up to 64 label arrays.
 constexpr int MS_FIRST = 4;
 constexpr int MS_GROUP = 8;
-constexpr int MS_GROUPS = BFS_THREADS / MS_GROUP;
+constexpr int MS_STRIDE = 4;
+constexpr int MS_EVENTS = 4;
+constexpr int MS_QBUF = 128;  // per-wave buffer of discovered vertices
+static_assert(MS_TILE == kWave, "a pull tile is one vertex per lane");
 
+struct MsPullWave {
+    int32_t vtx[kWave];    // pending vertex (by slot)
+    int32_t start[kWave];  // its next edge to scan
+    int32_t end[kWave];
+    unsigned long long want[kWave];
+    unsigned long long got[kWave];
+    uint32_t nev[kWave];
+    int32_t ev_u[MS_EVENTS][kWave];
+    unsigned long long ev_b[MS_EVENTS][kWave];
+    int32_t qbuf[MS_QBUF];
+};
 struct MsPullShared {
-    LdsKeyQueue q;
     int2 tab[MS_TILE_TABLE];  // (graph, first tile) of the graphs with bottom-up slots
-    int32_t pv[BFS_THREADS];  // unresolved vertex (tile-local index)
-    int32_t pe[BFS_THREADS];  // next edge to scan
-    int32_t pend_end[BFS_THREADS];
-    unsigned long long pwant[BFS_THREADS];
-    unsigned long long got[BFS_THREADS];
-    uint32_t npend;
+    MsPullWave w[BFS_WAVES];
+    uint32_t qn[BFS_WAVES];
+    uint32_t qbase[BFS_WAVES];
 };
 
 __device__ __forceinline__ bool ms_bm_test(const uint32_t* bm, int32_t u) { return (bm[u >> 5] >> (u & 31)) & 1u; }
 
-// OR over the MS_GROUP lanes of this lane's group, and over the group's
+// LDS written by some lanes of a wave and read by others after it: keep the
+// compiler from reordering across this point (a wave's LDS operations
+// complete in program order).
+__device__ __forceinline__ void ms_wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// OR over the W lanes of this lane's aligned group, and over the group's
 // lanes below this one.
+template <int W>
 __device__ __forceinline__ unsigned long long group_or(unsigned long long x) {
 #pragma unroll
-    for (int d = MS_GROUP / 2; d >= 1; d >>= 1) x |= __shfl_xor(x, d, kWave);
+    for (int d = W / 2; d >= 1; d >>= 1) x |= __shfl_xor(x, d, kWave);
     return x;
 }
+template <int W>
 __device__ __forceinline__ unsigned long long group_excl_or(unsigned long long x) {
-    const int sub = lane_id() & (MS_GROUP - 1);
+    const int sub = lane_id() & (W - 1);
 #pragma unroll
-    for (int d = 1; d < MS_GROUP; d <<= 1) {
+    for (int d = 1; d < W; d <<= 1) {
         const unsigned long long y = __shfl_up(x, d, kWave);
         if (sub >= d) x |= y;
     }
     const unsigned long long e = __shfl_up(x, 1, kWave);
     return sub == 0 ? 0ull : e;
+}
+
+// Tile t of the level: its first vertex, end and graph (-1: look up per
+// vertex).  Tiles never straddle graphs in table mode.
+__device__ __forceinline__ void ms_pull_tile(const MsArgs& a, const MsPullShared& s, bool table, int32_t ne,
+                                             uint32_t t, int32_t& base, int32_t& end, int32_t& tg) {
+    tg = -1;
+    if (table) {
+        int lo = 0, hi = ne - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if ((uint32_t)s.tab[mid].y <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        tg = s.tab[lo].x;
+        if ((uint32_t)tg < (uint32_t)a.G) {
+            base = a.vbase[tg] + (int32_t)(t - (uint32_t)s.tab[lo].y) * MS_TILE;
+            end = min(base + MS_TILE, a.vbase[tg + 1]);
+        } else {
+            if (lane_id() == 0) st_relaxed(&a.ctr->corrupt, 1u);
+            tg = -1;
+            base = end = 0;
+        }
+    } else {
+        base = (int32_t)t * MS_TILE;
+        end = min(base + MS_TILE, a.n);
+    }
 }
 
 __device__ void ms_pull_body(const MsArgs& a, int ph, MsPullShared& s) {
@@ -343,109 +401,206 @@ __device__ void ms_pull_body(const MsArgs& a, int ph, MsPullShared& s) {
     const bool table = ne <= MS_TILE_TABLE && ne <= a.G;
     for (int i = threadIdx.x; table && i < ne; i += BFS_THREADS) s.tab[i] = a.bu_list[(size_t)(ph & 1) * a.G + i];
     const uint32_t ntiles = table ? (uint32_t)tl : (uint32_t)(((int64_t)a.n + MS_TILE - 1) / MS_TILE);
-    if (threadIdx.x == 0) s.q.count = 0;
     __syncthreads();
-    const GlobalQueue out = ms_out_queue(a, ph);
     const unsigned long long* cur = a.front[ph % 3];
     unsigned long long* next = a.front[(ph + 1) % 3];
     const uint32_t* fbm = a.fbm[ph & 1];
-    const int tid = threadIdx.x;
-    const int grp = tid / MS_GROUP, sub = tid % MS_GROUP;
-    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-        int32_t base, end, tg = -1;
-        if (table) {
-            int lo = 0, hi = ne - 1;
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if ((uint32_t)s.tab[mid].y <= t) lo = mid;
-                else hi = mid - 1;
-            }
-            tg = s.tab[lo].x;
-            if ((uint32_t)tg < (uint32_t)a.G) {
-                base = a.vbase[tg] + (int32_t)(t - (uint32_t)s.tab[lo].y) * MS_TILE;
-                end = min(base + MS_TILE, a.vbase[tg + 1]);
-            } else {
-                if (tid == 0) st_relaxed(&a.ctr->corrupt, 1u);
-                tg = -1;
-                base = end = 0;
-            }
-        } else {
-            base = (int32_t)t * MS_TILE;
-            end = min(base + MS_TILE, a.n);
+    const int lane = lane_id();
+    const int wid = threadIdx.x / kWave;
+    const int grp = lane / MS_GROUP, sub = lane % MS_GROUP;
+    MsPullWave& W = s.w[wid];
+    const GlobalQueue out = ms_out_queue(a, ph);
+    uint32_t wq = 0;  // vertices in this wave's buffer (wave-uniform)
+
+    const uint32_t tstep = gridDim.x * BFS_WAVES;
+    uint32_t t = blockIdx.x * BFS_WAVES + wid;
+    // The next tile's masks and row offsets are loaded one tile ahead.
+    int32_t nbase = 0, nend = 0, ntg = -1;
+    unsigned long long nm = 0, nvis = 0;
+    int32_t noff0 = 0, noff1 = 0;
+    auto prefetch = [&](uint32_t tt) {
+        nm = 0;
+        if (tt >= ntiles) return;
+        ms_pull_tile(a, s, table, ne, tt, nbase, nend, ntg);
+        const int32_t v = nbase + lane;
+        if (v < nend) {
+            nm = a.bu[ntg >= 0 ? ntg : a.gid[v]];
+            nvis = a.visited[v];
+            noff0 = a.offs[v];
+            noff1 = a.offs[v + 1];
         }
-        if (tid == 0) s.npend = 0;
-        const int32_t v = base + tid;
-        unsigned long long want = 0, got = 0;
-        int32_t start = 0, deg = 0;
-        if (v < end) {
-            const unsigned long long m = a.bu[tg >= 0 ? tg : a.gid[v]];
-            if (m) {
-                want = m & ~a.visited[v];
-                if (want) {
-                    start = a.offs[v];
-                    deg = a.offs[v + 1] - start;
-                }
-            }
-        }
-        __syncthreads();  // npend reset visible; the previous tile is done with the LDS arrays
-        // Stage 1: the first MS_FIRST neighbours, loads issued together.
-        if (want != 0ull && deg > 0) {
+    };
+    prefetch(t);
+    for (; t < ntiles; t += tstep) {
+        const int32_t base = nbase, end = nend, tg = ntg;
+        const unsigned long long want = nm & ~nvis;
+        const int32_t start = noff0, deg = want ? noff1 - noff0 : 0;
+        prefetch(t + tstep);
+        if (__ballot(want != 0ull) == 0ull) continue;
+        const int32_t v = base + lane;
+
+        // Stage 1.
+        unsigned long long got = 0;
+        int32_t e1u[MS_FIRST];
+        unsigned long long e1b[MS_FIRST];
+        {
             int32_t u[MS_FIRST];
+            unsigned long long f[MS_FIRST];
 #pragma unroll
             for (int k = 0; k < MS_FIRST; ++k) u[k] = (k < deg) ? a.cols[start + k] : -1;
 #pragma unroll
+            for (int k = 0; k < MS_FIRST; ++k) f[k] = (u[k] >= 0 && ms_bm_test(fbm, u[k])) ? cur[u[k]] & want : 0ull;
+#pragma unroll
             for (int k = 0; k < MS_FIRST; ++k) {
-                if (u[k] >= 0 && ms_bm_test(fbm, u[k])) {
-                    const unsigned long long f = cur[u[k]] & want & ~got;
-                    if (f) {
-                        got |= f;
-                        ms_record(a, ph, v, u[k], f);
+                e1u[k] = u[k];
+                e1b[k] = f[k] & ~got;
+                got |= f[k];
+            }
+        }
+
+        // Stage 2: the wave's unresolved lists, one per MS_GROUP-lane group.
+        const bool pend = want != 0ull && got != want && deg > MS_FIRST;
+        const unsigned long long pm = __ballot(pend);
+        int slot = 0;
+        int nev2 = 0;
+        int32_t e2u[MS_EVENTS];
+        unsigned long long e2b[MS_EVENTS];
+#pragma unroll
+        for (int k = 0; k < MS_EVENTS; ++k) {
+            e2u[k] = -1;
+            e2b[k] = 0ull;
+        }
+        if (pm) {
+            slot = lane_prefix(pm);
+            if (pend) {
+                W.vtx[slot] = v;
+                W.start[slot] = start + MS_FIRST;
+                W.end[slot] = start + deg;
+                W.want[slot] = want;
+                W.got[slot] = got;
+                W.nev[slot] = 0u;
+            }
+            ms_wave_lds_sync();
+            const int np = __popcll(pm);
+            for (int i = grp; i < np; i += kWave / MS_GROUP) {
+                const int32_t e1 = W.end[i];
+                const unsigned long long w = W.want[i];
+                unsigned long long g = W.got[i];
+                for (int32_t e0 = W.start[i]; e0 < e1 && g != w; e0 += MS_GROUP * MS_STRIDE) {
+                    int32_t uu[MS_STRIDE];
+                    unsigned long long ff[MS_STRIDE];
+#pragma unroll
+                    for (int k = 0; k < MS_STRIDE; ++k) {
+                        const int32_t e = e0 + k * MS_GROUP + sub;
+                        uu[k] = e < e1 ? a.cols[e] : -1;
+                    }
+#pragma unroll
+                    for (int k = 0; k < MS_STRIDE; ++k)
+                        ff[k] = (uu[k] >= 0 && ms_bm_test(fbm, uu[k])) ? cur[uu[k]] & w : 0ull;
+#pragma unroll
+                    for (int k = 0; k < MS_STRIDE; ++k) {
+                        const unsigned long long f = ff[k] & ~g;
+                        const unsigned long long mine = f & ~group_excl_or<MS_GROUP>(f);
+                        if (mine) {
+                            const uint32_t n = atomicAdd(&W.nev[i], 1u);
+                            if (n < (uint32_t)MS_EVENTS) {
+                                W.ev_u[n][i] = uu[k];
+                                W.ev_b[n][i] = mine;
+                            } else {
+                                // Out of event slots: this credit's labels now.
+                                ms_record(a, ph, W.vtx[i], uu[k], mine);
+                            }
+                        }
+                        g |= group_or<MS_GROUP>(f);
+                    }
+                }
+                if (sub == 0) W.got[i] = g;
+            }
+            ms_wave_lds_sync();
+            if (pend) {
+                got = W.got[slot];
+                nev2 = (int)min(W.nev[slot], (uint32_t)MS_EVENTS);
+#pragma unroll
+                for (int k = 0; k < MS_EVENTS; ++k) {
+                    if (k < nev2) {
+                        e2u[k] = W.ev_u[k][slot];
+                        e2b[k] = W.ev_b[k][slot];
                     }
                 }
             }
-            if (got != want && deg > MS_FIRST) {
-                const uint32_t slot = atomicAdd(&s.npend, 1u);
-                s.pv[slot] = tid;
-                s.pe[slot] = start + MS_FIRST;
-                s.pend_end[slot] = start + deg;
-                s.pwant[slot] = want;
-            }
+            ms_wave_lds_sync();  // the next tile rewrites the scratch
         }
-        s.got[tid] = got;
-        __syncthreads();
-        // Stage 2: groups of MS_GROUP lanes scan the remaining lists.
-        const uint32_t np = s.npend;
-        for (uint32_t i = grp; i < np; i += MS_GROUPS) {
-            const int32_t lt = s.pv[i];
-            const int32_t vv = base + lt;
-            const int32_t e1 = s.pend_end[i];
-            const unsigned long long w = s.pwant[i];
-            unsigned long long g = s.got[lt];
-            for (int32_t e0 = s.pe[i]; e0 < e1 && g != w; e0 += MS_GROUP) {
-                const int32_t e = e0 + sub;
-                int32_t u = -1;
-                unsigned long long f = 0;
-                if (e < e1) {
-                    u = a.cols[e];
-                    if (ms_bm_test(fbm, u)) f = cur[u] & w & ~g;
-                }
-                const unsigned long long mine = f & ~group_excl_or(f);
-                if (mine) ms_record(a, ph, vv, u, mine);
-                g |= group_or(f);
-            }
-            if (sub == 0) s.got[lt] = g;
-        }
-        __syncthreads();
-        got = s.got[tid];
+
+        // Claim the new bits and buffer the vertices that join the next
+        // frontier.
         bool enq = false;
         if (got) {
             atomic_or_u64(&a.visited[v], got);
             enq = atomic_or_u64(&next[v], got) == 0ull;
         }
-        lq_append<false>(s.q, out, enq, v, 0);
-        lq_flush<false>(s.q, out, LQ_CAP / 2);
+        const unsigned long long em = __ballot(enq);
+        if (em) {
+            const uint32_t nf = (uint32_t)__popcll(em);
+            if (wq + nf > (uint32_t)MS_QBUF) {  // buffer full: flush this wave's part now
+                uint32_t qb0 = 0;
+                if (lane == 0) qb0 = atomic_add_u32(out.len, wq);
+                qb0 = (uint32_t)wave_bcast((int)qb0, 0);
+                for (uint32_t i = lane; i < wq; i += kWave) {
+                    if (qb0 + i < out.cap) out.keys[qb0 + i] = W.qbuf[i];
+                    else st_relaxed(out.overflow, 1u);
+                }
+                wq = 0;
+                ms_wave_lds_sync();
+            }
+            if (enq) W.qbuf[wq + (uint32_t)lane_prefix(em)] = v;
+            wq += nf;
+        }
+
+        // Labels, one query (bit) at a time.
+        unsigned long long staged = 0;
+#pragma unroll
+        for (int k = 0; k < MS_FIRST; ++k) staged |= e1b[k];
+#pragma unroll
+        for (int k = 0; k < MS_EVENTS; ++k) staged |= e2b[k];
+        unsigned long long bits = group_or<kWave>(staged);
+        while (bits) {
+            const int b = __ffsll(bits) - 1;
+            bits &= bits - 1;
+            if (!((staged >> b) & 1ull)) continue;
+            int32_t pu = -1;
+#pragma unroll
+            for (int k = 0; k < MS_FIRST; ++k)
+                if ((e1b[k] >> b) & 1ull) pu = e1u[k];
+#pragma unroll
+            for (int k = 0; k < MS_EVENTS; ++k)
+                if ((e2b[k] >> b) & 1ull) pu = e2u[k];
+            const int32_t g = tg >= 0 ? tg : a.gid[v];
+            const int32_t q = a.bitq[g * 64 + b];
+            if (q >= 0) {
+                const long long idx = a.lbase[q] + (v - a.vbase[g]);
+                a.depth_out[idx] = ph + 1 - a.qstart[q];
+                a.pred_out[idx] = pu - a.vbase[g];
+            }
+        }
     }
-    lq_flush<false>(s.q, out, 1);
+    // One queue reservation for the whole workgroup's buffered vertices.
+    if (lane == 0) s.qn[wid] = wq;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t total = 0;
+        for (int w = 0; w < BFS_WAVES; ++w) {
+            s.qbase[w] = total;
+            total += s.qn[w];
+        }
+        const uint32_t qb0 = total ? atomic_add_u32(out.len, total) : 0u;
+        for (int w = 0; w < BFS_WAVES; ++w) s.qbase[w] += qb0;
+    }
+    __syncthreads();
+    for (uint32_t i = lane; i < wq; i += kWave) {
+        const uint32_t pos = s.qbase[wid] + i;
+        if (pos < out.cap) out.keys[pos] = W.qbuf[i];
+        else st_relaxed(out.overflow, 1u);
+    }
 }
 
 // Grid tier of level ph's push: workgroups take chunks grid-stride, one edge
