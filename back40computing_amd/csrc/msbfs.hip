@@ -145,7 +145,43 @@ struct MsShared {
 
 // Labels for every newly reached (query, v) bit; `src` is the discovering
 // neighbour (global id) or -1 for a source.
+// Up to four bits at a time, so their slot lookups (query, then its label
+// base and start level) are in flight together.
 __device__ __forceinline__ void ms_record(const MsArgs& a, int ph, int32_t v, int32_t src, unsigned long long bits) {
+    const int32_t g = a.gid[v];
+    const int32_t vb = a.vbase[g];
+    while (bits) {
+        int32_t q[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            q[k] = -1;
+            if (bits) {
+                const int b = __ffsll(bits) - 1;
+                bits &= bits - 1;
+                q[k] = a.bitq[g * 64 + b];
+            }
+        }
+        long long lb[4];
+        int32_t q0[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            lb[k] = q[k] >= 0 ? a.lbase[q[k]] : 0;
+            q0[k] = q[k] >= 0 ? a.qstart[q[k]] : 0;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            if (q[k] < 0) continue;
+            const long long idx = lb[k] + (v - vb);
+            a.depth_out[idx] = ph + 1 - q0[k];
+            a.pred_out[idx] = src - vb;
+        }
+    }
+}
+
+// One bit at a time: the pull's rare path (a vertex out of credit-event
+// slots), small so that it costs the pull few registers.
+__device__ __forceinline__ void ms_record_rare(const MsArgs& a, int ph, int32_t v, int32_t src,
+                                               unsigned long long bits) {
     const int32_t g = a.gid[v];
     const int32_t vb = a.vbase[g];
     while (bits) {
@@ -411,6 +447,9 @@ __device__ void ms_pull_body(const MsArgs& a, int ph, MsPullShared& s) {
     MsPullWave& W = s.w[wid];
     const GlobalQueue out = ms_out_queue(a, ph);
     uint32_t wq = 0;  // vertices in this wave's buffer (wave-uniform)
+    int32_t cg = -1;   // graph whose slot labels the lanes hold (slb, sdv)
+    long long slb = 0;
+    int32_t sdv = -1;
 
     const uint32_t tstep = gridDim.x * BFS_WAVES;
     uint32_t t = blockIdx.x * BFS_WAVES + wid;
@@ -508,7 +547,7 @@ __device__ void ms_pull_body(const MsArgs& a, int ph, MsPullShared& s) {
                                 W.ev_b[n][i] = mine;
                             } else {
                                 // Out of event slots: this credit's labels now.
-                                ms_record(a, ph, W.vtx[i], uu[k], mine);
+                                ms_record_rare(a, ph, W.vtx[i], uu[k], mine);
                             }
                         }
                         g |= group_or<MS_GROUP>(f);
@@ -562,11 +601,33 @@ __device__ void ms_pull_body(const MsArgs& a, int ph, MsPullShared& s) {
         for (int k = 0; k < MS_FIRST; ++k) staged |= e1b[k];
 #pragma unroll
         for (int k = 0; k < MS_EVENTS; ++k) staged |= e2b[k];
+        // Lane b holds bit b's label offset and depth value for the tile's
+        // graph (reloaded when the graph changes), read back per bit with a
+        // lane read instead of two dependent loads.
+        if (tg >= 0 && tg != cg) {
+            cg = tg;
+            const int32_t q = a.bitq[tg * 64 + lane];
+            slb = q >= 0 ? a.lbase[q] - a.vbase[tg] : 0ll;  // may be negative
+            sdv = q >= 0 ? ph + 1 - a.qstart[q] : -1;      // -1: no query in slot
+        }
         unsigned long long bits = group_or<kWave>(staged);
         while (bits) {
-            const int b = __ffsll(bits) - 1;
+            const int b = __builtin_amdgcn_readfirstlane(__ffsll(bits) - 1);
             bits &= bits - 1;
-            if (!((staged >> b) & 1ull)) continue;
+            long long lb;
+            int32_t dv, gv;
+            if (tg >= 0) {
+                lb = (long long)(((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)(slb >> 32), b) << 32) |
+                                 (uint32_t)__builtin_amdgcn_readlane((int)slb, b));
+                dv = __builtin_amdgcn_readlane(sdv, b);
+                gv = tg;
+            } else {
+                gv = v < end ? a.gid[v] : 0;
+                const int32_t q = a.bitq[gv * 64 + b];
+                lb = q >= 0 ? a.lbase[q] - a.vbase[gv] : 0ll;
+                dv = q >= 0 ? ph + 1 - a.qstart[q] : -1;
+            }
+            if (!((staged >> b) & 1ull) || dv < 0) continue;
             int32_t pu = -1;
 #pragma unroll
             for (int k = 0; k < MS_FIRST; ++k)
@@ -574,13 +635,8 @@ __device__ void ms_pull_body(const MsArgs& a, int ph, MsPullShared& s) {
 #pragma unroll
             for (int k = 0; k < MS_EVENTS; ++k)
                 if ((e2b[k] >> b) & 1ull) pu = e2u[k];
-            const int32_t g = tg >= 0 ? tg : a.gid[v];
-            const int32_t q = a.bitq[g * 64 + b];
-            if (q >= 0) {
-                const long long idx = a.lbase[q] + (v - a.vbase[g]);
-                a.depth_out[idx] = ph + 1 - a.qstart[q];
-                a.pred_out[idx] = pu - a.vbase[g];
-            }
+            a.depth_out[lb + v] = dv;
+            a.pred_out[lb + v] = pu - a.vbase[gv];
         }
     }
     // One queue reservation for the whole workgroup's buffered vertices.
