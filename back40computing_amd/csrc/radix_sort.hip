@@ -84,10 +84,17 @@ __global__ __launch_bounds__(RS_THREADS) void k_upsweep(const K* k0, const K* k1
     const K* keys = *sel ? k1 : k0;
     const int64_t begin = (int64_t)blockIdx.x * keys_per_wg;
     const int64_t end = min(n, begin + keys_per_wg);
-    for (int64_t i = begin + tid; i < end; i += RS_THREADS) {
-        uint32_t d = digit_of<K, CONV>(keys[i], shift, mask);
-        atomicAdd(&hist[wid][d], 1u);
+    // RS_UP keys per thread per round, their loads issued together.
+    constexpr int RS_UP = 8;
+    int64_t i = begin + tid;
+    for (; i + (int64_t)(RS_UP - 1) * RS_THREADS < end; i += (int64_t)RS_UP * RS_THREADS) {
+        K kk[RS_UP];
+#pragma unroll
+        for (int j = 0; j < RS_UP; ++j) kk[j] = keys[i + (int64_t)j * RS_THREADS];
+#pragma unroll
+        for (int j = 0; j < RS_UP; ++j) atomicAdd(&hist[wid][digit_of<K, CONV>(kk[j], shift, mask)], 1u);
     }
+    for (; i < end; i += RS_THREADS) atomicAdd(&hist[wid][digit_of<K, CONV>(keys[i], shift, mask)], 1u);
     __syncthreads();
     uint32_t c = 0;
 #pragma unroll
@@ -152,6 +159,21 @@ __global__ __launch_bounds__(RS_THREADS) void k_downsweep(K* k0, K* k1, uint32_t
     const int64_t end = min(n, begin + keys_per_wg);
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 
+    // The next tile's keys (and values) are loaded into registers while the
+    // current tile is ranked and scattered.
+    K nkey[ITEMS];
+    uint32_t nval[ITEMS];
+    auto load_tile = [&](int64_t tb) {
+        const int tn = (int)min((int64_t)RS_TILE, end - tb);
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
+            const int idx = wid * (kWave * ITEMS) + j * kWave + lane;
+            const bool valid = idx < tn;
+            nkey[j] = valid ? kin[tb + idx] : (K)0;
+            if (VALS) nval[j] = valid ? vin[tb + idx] : 0u;
+        }
+    };
+    if (begin < end) load_tile(begin);
     for (int64_t tile_base = begin; tile_base < end; tile_base += RS_TILE) {
         const int tile_n = (int)min((int64_t)RS_TILE, end - tile_base);
         for (int i = lane; i < RS_BINS; i += kWave) wave_hist[wid][i] = 0;
@@ -163,10 +185,14 @@ __global__ __launch_bounds__(RS_THREADS) void k_downsweep(K* k0, K* k1, uint32_t
         uint32_t dig[ITEMS];
 #pragma unroll
         for (int j = 0; j < ITEMS; ++j) {
+            key[j] = nkey[j];
+            if (VALS) val[j] = nval[j];
+        }
+        if (tile_base + RS_TILE < end) load_tile(tile_base + RS_TILE);
+#pragma unroll
+        for (int j = 0; j < ITEMS; ++j) {
             const int idx = wid * (kWave * ITEMS) + j * kWave + lane;
             const bool valid = idx < tile_n;
-            key[j] = valid ? kin[tile_base + idx] : (K)0;
-            if (VALS) val[j] = valid ? vin[tile_base + idx] : 0u;
             const uint32_t d = digit_of<K, CONV>(key[j], shift, mask);
             dig[j] = d;
             uint64_t peers = __ballot(valid);
