@@ -21,9 +21,11 @@
 //  * k_ms_push + k_ms_heavy: top-down push over the frontier queue for the
 //    bits in top-down mode (64-bit atomic ORs); long lists go to a
 //    load-balanced grid tier.
-//  * k_ms_pull: bottom-up pull over the vertices of graphs with bits in
-//    bottom-up mode (each vertex scans its neighbours until every wanted bit
-//    is found; unresolved lists are scanned by 8-lane groups).  A young
+//  * k_ms_pull_narrow + k_ms_pull: bottom-up pull over the vertices of
+//    graphs with bits in bottom-up mode (each vertex scans its neighbours
+//    until every wanted bit is found; unresolved lists are scanned by 8-lane
+//    groups), in a narrow form for graphs with a few such bits and a wide
+//    form (staged, coalesced label stores) for graphs with many.  A young
 //    query's bit is never in a bottom-up want set, so the pull's early exit
 //    keeps working while queries of every age share it.
 //  * k_ms_count: per (graph, slot) count and degree sum of the vertices
@@ -51,7 +53,8 @@ struct alignas(256) MsCounters {
     uint32_t bar_abort;  // persistent form: a grid-barrier wait timed out
     uint32_t pad[7];
     unsigned long long heavy_alloc[2];  // (entries << 32 | chunks) deferred to the grid tier, by parity
-    unsigned long long bu_tiles[2];     // (graphs << 32 | tiles) listed for the pull, by parity
+    unsigned long long bu_tiles[2];     // (graphs << 32 | tiles) listed for the wide pull, by parity
+    unsigned long long bu_tiles_n[2];   // the same for the narrow pull
     unsigned long long bar_release;     // persistent form: grid-barrier release word (epoch | payload)
 };
 
@@ -76,7 +79,7 @@ struct MsArgs {
     int32_t* pred_out;
     int32_t* queue[3];                 // frontier queues, rotating by level % 3 (front[] pairs with queue[])
     uint32_t* fbm[2];                  // [ceil(n/32)] by level parity: vertices carrying any bit in front[]
-    int2* bu_list;                     // [2][G] by level parity: (graph, first tile) of graphs with bottom-up slots
+    int2* bu_list;                     // [2 classes][2 parities][G]: (graph, first tile) of graphs with bottom-up slots
     MsCounters* hctr;                  // device address of the pinned host mirror (overflow, corrupt)
     int32_t* hdone;                    // [Q] device address of pinned host words: levels of a finished query
     int32_t* dph;                      // captured level loops: phase of the replay's first level (else null)
@@ -134,7 +137,11 @@ constexpr int MS_SPARSE_DIV = 16;
 // The pull walks the tiles of at most this many graphs through an LDS table
 // (more: every tile of the pack).
 constexpr int MS_TILE_TABLE = 512;
-constexpr int MS_TILE = 64;  // vertices per pull tile
+constexpr int MS_TILE = 64;          // vertices per wide-pull tile (one per lane of a wave)
+constexpr int MS_TILE_N = BFS_THREADS;  // vertices per narrow-pull tile (one per thread of a workgroup)
+// A graph pulls in the wide form when more than this many of its slots are
+// in bottom-up mode, else in the narrow form.
+constexpr int MS_WIDE_BITS = 8;
 constexpr int kMsMaxFusedGrid = 4096;  // grid-barrier arrival words of the persistent form
 
 struct MsShared {
@@ -178,9 +185,10 @@ __device__ __forceinline__ void ms_record(const MsArgs& a, int ph, int32_t v, in
     }
 }
 
-// One bit at a time: the pull's rare path (a vertex out of credit-event
-// slots), small so that it costs the pull few registers.
-__device__ __forceinline__ void ms_record_rare(const MsArgs& a, int ph, int32_t v, int32_t src,
+// One bit at a time: small code, for the pulls (the wide form's rare path,
+// a vertex out of credit-event slots, and the narrow form, whose vertices
+// carry a few bits).
+__device__ __forceinline__ void ms_record_one(const MsArgs& a, int ph, int32_t v, int32_t src,
                                                unsigned long long bits) {
     const int32_t g = a.gid[v];
     const int32_t vb = a.vbase[g];
@@ -286,6 +294,7 @@ __device__ void ms_push_body(const MsArgs& a, int ph, MsShared& s) {
         st_relaxed(&a.ctr->qlen[(ph + 2) & 3], 0u);
         __hip_atomic_store(&a.ctr->heavy_alloc[(ph + 1) & 1], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(&a.ctr->bu_tiles[(ph + 1) & 1], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&a.ctr->bu_tiles_n[(ph + 1) & 1], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     if (n_f == 0) return;  // no vertex carries a bit: nothing can be found
     if (threadIdx.x == 0) s.q.count = 0;
@@ -342,6 +351,7 @@ __device__ void ms_push_body(const MsArgs& a, int ph, MsShared& s) {
 // per vertex over up to 64 label arrays.
 constexpr int MS_FIRST = 4;
 constexpr int MS_GROUP = 8;
+constexpr int MS_GROUPS = BFS_THREADS / MS_GROUP;
 constexpr int MS_STRIDE = 4;
 constexpr int MS_EVENTS = 4;
 constexpr int MS_QBUF = 128;  // per-wave buffer of discovered vertices
@@ -427,15 +437,20 @@ __device__ void ms_pull_body(const MsArgs& a, int ph, MsPullShared& s) {
     uint32_t n_f = ld_relaxed(&a.ctr->qlen[ph & 3]);
     if (n_f > a.qcap) n_f = 0;
     if (n_f == 0 || !ld_relaxed(&a.ctr->any_bu[ph & 1])) return;
-    // Tiles: with the level's graph table (k_ms_direction lists every graph
-    // with bottom-up slots, in tile order), tile t lies in graph tab[e].x for
-    // the last e with tab[e].y <= t; else tile t is vertices [t, t+1) * MS_TILE
-    // of the pack.
+    // Tiles: with the level's wide-form graph table (k_ms_direction lists
+    // every graph with more than MS_WIDE_BITS bottom-up slots, in tile
+    // order), tile t lies in graph tab[e].x for the last e with
+    // tab[e].y <= t.  A table too long for LDS: tile t is vertices
+    // [t, t+1) * MS_TILE of the pack, with every graph's bottom-up bits; a
+    // graph of the narrow form is then pulled twice, the second pass finding
+    // only bits the first could not (correct, just slower).
     const unsigned long long tl =
         __hip_atomic_load(&a.ctr->bu_tiles[ph & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const int32_t ne = (int32_t)(tl >> 32);
     const bool table = ne <= MS_TILE_TABLE && ne <= a.G;
-    for (int i = threadIdx.x; table && i < ne; i += BFS_THREADS) s.tab[i] = a.bu_list[(size_t)(ph & 1) * a.G + i];
+    if (ne == 0) return;  // no graph pulls in the wide form
+    for (int i = threadIdx.x; table && i < ne; i += BFS_THREADS)
+        s.tab[i] = a.bu_list[(size_t)(2 + (ph & 1)) * a.G + i];
     const uint32_t ntiles = table ? (uint32_t)tl : (uint32_t)(((int64_t)a.n + MS_TILE - 1) / MS_TILE);
     __syncthreads();
     const unsigned long long* cur = a.front[ph % 3];
@@ -547,7 +562,7 @@ __device__ void ms_pull_body(const MsArgs& a, int ph, MsPullShared& s) {
                                 W.ev_b[n][i] = mine;
                             } else {
                                 // Out of event slots: this credit's labels now.
-                                ms_record_rare(a, ph, W.vtx[i], uu[k], mine);
+                                ms_record_one(a, ph, W.vtx[i], uu[k], mine);
                             }
                         }
                         g |= group_or<MS_GROUP>(f);
@@ -657,6 +672,145 @@ __device__ void ms_pull_body(const MsArgs& a, int ph, MsPullShared& s) {
         if (pos < out.cap) out.keys[pos] = W.qbuf[i];
         else st_relaxed(out.overflow, 1u);
     }
+}
+
+// Narrow form of the pull, for graphs with at most MS_WIDE_BITS slots in
+// bottom-up mode: workgroups take tiles of MS_TILE_N consecutive vertices
+// (one per thread); stage 1 checks each vertex's first MS_FIRST neighbours,
+// stage 2 scans the unresolved lists with MS_GROUP-lane groups, one
+// neighbour per lane per step, and every credit's labels are written as it
+// is found (few bits per vertex: the stores stay few).  At 8 waves per SIMD
+// it beats the wide form when most vertices want only a few bits.
+struct MsPullNarrowShared {
+    LdsKeyQueue q;
+    int2 tab[MS_TILE_TABLE];  // (graph, first tile) of the graphs with bottom-up slots
+    int32_t pv[BFS_THREADS];  // unresolved vertex (tile-local index)
+    int32_t pe[BFS_THREADS];  // next edge to scan
+    int32_t pend_end[BFS_THREADS];
+    unsigned long long pwant[BFS_THREADS];
+    unsigned long long got[BFS_THREADS];
+    uint32_t npend;
+};
+
+__device__ void ms_pull_narrow_body(const MsArgs& a, int ph, MsPullNarrowShared& s) {
+    uint32_t n_f = ld_relaxed(&a.ctr->qlen[ph & 3]);
+    if (n_f > a.qcap) n_f = 0;
+    if (n_f == 0 || !ld_relaxed(&a.ctr->any_bu[ph & 1])) return;
+    // Tiles: with the level's narrow-form graph table (graphs with at most
+    // MS_WIDE_BITS bottom-up slots, in tile order), tile t lies in graph
+    // tab[e].x for the last e with tab[e].y <= t.  A table too long for LDS:
+    // tile t is vertices [t, t+1) * MS_TILE_N of the pack, with every graph's
+    // bottom-up bits (the wide pull that follows then finds only what is
+    // left of its graphs).
+    const unsigned long long tl =
+        __hip_atomic_load(&a.ctr->bu_tiles_n[ph & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const int32_t ne = (int32_t)(tl >> 32);
+    const bool table = ne <= MS_TILE_TABLE && ne <= a.G;
+    for (int i = threadIdx.x; table && i < ne; i += BFS_THREADS) s.tab[i] = a.bu_list[(size_t)(ph & 1) * a.G + i];
+    const uint32_t ntiles = table ? (uint32_t)tl : (uint32_t)(((int64_t)a.n + MS_TILE_N - 1) / MS_TILE_N);
+    if (threadIdx.x == 0) s.q.count = 0;
+    __syncthreads();
+    const GlobalQueue out = ms_out_queue(a, ph);
+    const unsigned long long* cur = a.front[ph % 3];
+    unsigned long long* next = a.front[(ph + 1) % 3];
+    const uint32_t* fbm = a.fbm[ph & 1];
+    const int tid = threadIdx.x;
+    const int grp = tid / MS_GROUP, sub = tid % MS_GROUP;
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        int32_t base, end, tg = -1;
+        if (table) {
+            int lo = 0, hi = ne - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if ((uint32_t)s.tab[mid].y <= t) lo = mid;
+                else hi = mid - 1;
+            }
+            tg = s.tab[lo].x;
+            if ((uint32_t)tg < (uint32_t)a.G) {
+                base = a.vbase[tg] + (int32_t)(t - (uint32_t)s.tab[lo].y) * MS_TILE_N;
+                end = min(base + MS_TILE_N, a.vbase[tg + 1]);
+            } else {
+                if (tid == 0) st_relaxed(&a.ctr->corrupt, 1u);
+                tg = -1;
+                base = end = 0;
+            }
+        } else {
+            base = (int32_t)t * MS_TILE_N;
+            end = min(base + MS_TILE_N, a.n);
+        }
+        if (tid == 0) s.npend = 0;
+        const int32_t v = base + tid;
+        unsigned long long want = 0, got = 0;
+        int32_t start = 0, deg = 0;
+        if (v < end) {
+            const unsigned long long m = a.bu[tg >= 0 ? tg : a.gid[v]];
+            if (m) {
+                want = m & ~a.visited[v];
+                if (want) {
+                    start = a.offs[v];
+                    deg = a.offs[v + 1] - start;
+                }
+            }
+        }
+        __syncthreads();  // npend reset visible; the previous tile is done with the LDS arrays
+        // Stage 1: the first MS_FIRST neighbours, loads issued together.
+        if (want != 0ull && deg > 0) {
+            int32_t u[MS_FIRST];
+#pragma unroll
+            for (int k = 0; k < MS_FIRST; ++k) u[k] = (k < deg) ? a.cols[start + k] : -1;
+#pragma unroll
+            for (int k = 0; k < MS_FIRST; ++k) {
+                if (u[k] >= 0 && ms_bm_test(fbm, u[k])) {
+                    const unsigned long long f = cur[u[k]] & want & ~got;
+                    if (f) {
+                        got |= f;
+                        ms_record_one(a, ph, v, u[k], f);
+                    }
+                }
+            }
+            if (got != want && deg > MS_FIRST) {
+                const uint32_t slot = atomicAdd(&s.npend, 1u);
+                s.pv[slot] = tid;
+                s.pe[slot] = start + MS_FIRST;
+                s.pend_end[slot] = start + deg;
+                s.pwant[slot] = want;
+            }
+        }
+        s.got[tid] = got;
+        __syncthreads();
+        // Stage 2: groups of MS_GROUP lanes scan the remaining lists.
+        const uint32_t np = s.npend;
+        for (uint32_t i = grp; i < np; i += MS_GROUPS) {
+            const int32_t lt = s.pv[i];
+            const int32_t vv = base + lt;
+            const int32_t e1 = s.pend_end[i];
+            const unsigned long long w = s.pwant[i];
+            unsigned long long g = s.got[lt];
+            for (int32_t e0 = s.pe[i]; e0 < e1 && g != w; e0 += MS_GROUP) {
+                const int32_t e = e0 + sub;
+                int32_t u = -1;
+                unsigned long long f = 0;
+                if (e < e1) {
+                    u = a.cols[e];
+                    if (ms_bm_test(fbm, u)) f = cur[u] & w & ~g;
+                }
+                const unsigned long long mine = f & ~group_excl_or<MS_GROUP>(f);
+                if (mine) ms_record_one(a, ph, vv, u, mine);
+                g |= group_or<MS_GROUP>(f);
+            }
+            if (sub == 0) s.got[lt] = g;
+        }
+        __syncthreads();
+        got = s.got[tid];
+        bool enq = false;
+        if (got) {
+            atomic_or_u64(&a.visited[v], got);
+            enq = atomic_or_u64(&next[v], got) == 0ull;
+        }
+        lq_append<false>(s.q, out, enq, v, 0);
+        lq_flush<false>(s.q, out, LQ_CAP / 2);
+    }
+    lq_flush<false>(s.q, out, 1);
 }
 
 // Grid tier of level ph's push: workgroups take chunks grid-stride, one edge
@@ -835,16 +989,20 @@ __device__ void ms_direction_body(const MsArgs& a, int ph) {
             a.bu[g] = bum;
             if (done) atomic_and_u64(&a.active[g], ~done);
             if (bum) {
-                // List the graph for the next level's pull: one 64-bit atomic
-                // gives both its table entry and its first tile, so the table
-                // is in tile order.
+                // List the graph for the next level's pull, in the wide or
+                // the narrow table by its bottom-up slot count: one 64-bit
+                // atomic gives both its table entry and its first tile, so
+                // each table is in tile order.
                 st_relaxed(&a.ctr->any_bu[(ph + 1) & 1], 1u);
-                const uint32_t nt = (uint32_t)((Vg + MS_TILE - 1) / MS_TILE);
+                const int cls = __popcll(bum) > MS_WIDE_BITS ? 1 : 0;
+                const int tile = cls ? MS_TILE : MS_TILE_N;
+                const uint32_t nt = (uint32_t)((Vg + tile - 1) / tile);
                 const unsigned long long t = __hip_atomic_fetch_add(
-                    &a.ctr->bu_tiles[(ph + 1) & 1], (1ull << 32) | (unsigned long long)nt, __ATOMIC_RELAXED,
-                    __HIP_MEMORY_SCOPE_AGENT);
+                    cls ? &a.ctr->bu_tiles[(ph + 1) & 1] : &a.ctr->bu_tiles_n[(ph + 1) & 1],
+                    (1ull << 32) | (unsigned long long)nt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 const uint32_t e = (uint32_t)(t >> 32);
-                if (e < (uint32_t)a.G) a.bu_list[(size_t)((ph + 1) & 1) * a.G + e] = make_int2(g, (int)(uint32_t)t);
+                if (e < (uint32_t)a.G)
+                    a.bu_list[(size_t)(cls * 2 + ((ph + 1) & 1)) * a.G + e] = make_int2(g, (int)(uint32_t)t);
             }
         }
     }
@@ -872,19 +1030,24 @@ __global__ __launch_bounds__(BFS_THREADS) void k_ms_pull(MsArgs a, int ph) {
     __shared__ MsPullShared s;
     ms_pull_body(a, ms_phase(a, ph), s);
 }
+__global__ __launch_bounds__(BFS_THREADS) void k_ms_pull_narrow(MsArgs a, int ph) {
+    __shared__ MsPullNarrowShared s;
+    ms_pull_narrow_body(a, ms_phase(a, ph), s);
+}
 __global__ __launch_bounds__(BFS_THREADS) void k_ms_count(MsArgs a, int ph) { ms_count_body(a, ms_phase(a, ph)); }
 __global__ __launch_bounds__(BFS_THREADS) void k_ms_direction(MsArgs a, int ph) {
     ms_direction_body(a, ms_phase(a, ph));
 }
 
 // Persistent form: `levels` whole levels in one launch of a co-resident grid,
-// the five steps of a level separated by grid barriers (fenced mode: the
+// the six steps of a level separated by grid barriers (fenced mode: the
 // steps pass masks, queues and counters through plain accesses).  Queries
 // that finish retire on the device as in the other forms; a level after the
 // last query finished costs only its barriers.
 union MsFusedShared {
     MsShared push;
     MsPullShared pull;
+    MsPullNarrowShared pull_n;
 };
 __global__ __launch_bounds__(BFS_THREADS) void k_ms_fused(MsArgs a, int ph0, int levels) {
     __shared__ MsFusedShared s;
@@ -896,6 +1059,8 @@ __global__ __launch_bounds__(BFS_THREADS) void k_ms_fused(MsArgs a, int ph0, int
         ms_push_body(a, ph, s.push);
         if (!grid_sync(bar, ++epoch, &s_word)) break;
         ms_heavy_body(a, ph, s.push);
+        if (!grid_sync(bar, ++epoch, &s_word)) break;
+        ms_pull_narrow_body(a, ph, s.pull_n);
         if (!grid_sync(bar, ++epoch, &s_word)) break;
         ms_pull_body(a, ph, s.pull);
         if (!grid_sync(bar, ++epoch, &s_word)) break;
@@ -1109,7 +1274,7 @@ struct b40c_msbfs {
     int32_t* h_done = nullptr;      // pinned [qmax]: levels of a query once finished (0 while running)
     int32_t* h_done_dev = nullptr;  // its device address
     int32_t qmax = 0;
-    // Captured level loop (levels_per_replay > 0): the five level launches of
+    // Captured level loop (levels_per_replay > 0): the six level launches of
     // `levels_per_replay` levels and k_ms_advance, replayed as one graph.
     int32_t levels_per_replay = 0;
     bool persistent = false;             // levels_per_replay levels per step in one k_ms_fused launch
@@ -1297,11 +1462,12 @@ b40c_status ms_retire(b40c_msbfs* m) {
     return B40C_OK;
 }
 
-// The five launches of one level (phase argument `ph`: absolute, or the
+// The six launches of one level (phase argument `ph`: absolute, or the
 // level's index within a captured replay when a.dph is set).
 void ms_launch_level(b40c_msbfs* m, const MsArgs& a, int ph) {
     hipLaunchKernelGGL(k_ms_push, dim3(m->grid), dim3(BFS_THREADS), 0, m->stream, a, ph);
     hipLaunchKernelGGL(k_ms_heavy, dim3(m->grid), dim3(BFS_THREADS), 0, m->stream, a, ph);
+    hipLaunchKernelGGL(k_ms_pull_narrow, dim3(2 * m->grid), dim3(BFS_THREADS), 0, m->stream, a, ph);
     hipLaunchKernelGGL(k_ms_pull, dim3(2 * m->grid), dim3(BFS_THREADS), 0, m->stream, a, ph);
     hipLaunchKernelGGL(k_ms_count, dim3(m->grid), dim3(BFS_THREADS), 0, m->stream, a, ph);
     hipLaunchKernelGGL(k_ms_direction, dim3((m->G + BFS_WAVES - 1) / BFS_WAVES), dim3(BFS_THREADS), 0, m->stream, a,
@@ -1487,7 +1653,7 @@ b40c_status b40c_msbfs_create(b40c_multigraph* mg, int64_t label_budget_bytes, i
     if (s == B40C_OK) s = m->depth.alloc((size_t)m->arena_elems * 4);
     if (s == B40C_OK) s = m->pred.alloc((size_t)m->arena_elems * 4);
     for (int i = 0; i < 3 && s == B40C_OK; ++i) s = m->queue[i].alloc(N * 4);
-    if (s == B40C_OK) s = m->bu_list.alloc((size_t)m->G * 2 * sizeof(int2));
+    if (s == B40C_OK) s = m->bu_list.alloc((size_t)m->G * 4 * sizeof(int2));
     if (s == B40C_OK) s = m->counters.alloc(sizeof(MsCounters));
     m->heavy_cap = (uint32_t)m->n;
     m->chunk_cap = (uint32_t)std::min<int64_t>(mg->packed->edges / HEAVY_CHUNK + (int64_t)m->n + 64, 0x7FFFFFFFll);

@@ -301,14 +301,14 @@ b40c_status b40c_msbfs_set_tuning(b40c_msbfs* m, int32_t alpha, int32_t beta, in
 b40c_status b40c_msbfs_set_admission(b40c_msbfs* m, int32_t cohort_gap);
 /* Launch form of a step (north-star batched form; no counterpart in the
  * reference).  levels >= 1: each b40c_msbfs_step replays one hipGraph that
- * holds `levels` levels (five kernels each, the phase read from device
+ * holds `levels` levels (six kernels each, the phase read from device
  * memory) and one phase-advance node, then synchronizes once; queries that
  * finish inside the replay are retired on the device and reported at its
  * end.  0 (default): each step launches one level's kernels directly. */
 b40c_status b40c_msbfs_set_levels_per_step(b40c_msbfs* m, int32_t levels);
 /* Persistent form (north-star batched form; no counterpart in the
  * reference): on = 1 runs each step's levels (levels_per_step, at least 1)
- * in one launch of a co-resident grid, the five steps of every level
+ * in one launch of a co-resident grid, the six steps of every level
  * separated by software grid barriers; 0 (default) uses launches or the
  * captured graph.  The grid is the occupancy API's co-resident maximum
  * capped by the tuning's grid; a barrier wait that times out (workgroups not
