@@ -151,45 +151,10 @@ struct MsShared {
 };
 
 // Labels for every newly reached (query, v) bit; `src` is the discovering
-// neighbour (global id) or -1 for a source.
-// Up to four bits at a time, so their slot lookups (query, then its label
-// base and start level) are in flight together.
+// neighbour (global id) or -1 for a source.  One bit at a time: unrolling
+// the slot lookups four bits at a time measured the same and doubled the
+// pull's code.
 __device__ __forceinline__ void ms_record(const MsArgs& a, int ph, int32_t v, int32_t src, unsigned long long bits) {
-    const int32_t g = a.gid[v];
-    const int32_t vb = a.vbase[g];
-    while (bits) {
-        int32_t q[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            q[k] = -1;
-            if (bits) {
-                const int b = __ffsll(bits) - 1;
-                bits &= bits - 1;
-                q[k] = a.bitq[g * 64 + b];
-            }
-        }
-        long long lb[4];
-        int32_t q0[4];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            lb[k] = q[k] >= 0 ? a.lbase[q[k]] : 0;
-            q0[k] = q[k] >= 0 ? a.qstart[q[k]] : 0;
-        }
-#pragma unroll
-        for (int k = 0; k < 4; ++k) {
-            if (q[k] < 0) continue;
-            const long long idx = lb[k] + (v - vb);
-            a.depth_out[idx] = ph + 1 - q0[k];
-            a.pred_out[idx] = src - vb;
-        }
-    }
-}
-
-// One bit at a time: small code, for the pulls (the wide form's rare path,
-// a vertex out of credit-event slots, and the narrow form, whose vertices
-// carry a few bits).
-__device__ __forceinline__ void ms_record_one(const MsArgs& a, int ph, int32_t v, int32_t src,
-                                               unsigned long long bits) {
     const int32_t g = a.gid[v];
     const int32_t vb = a.vbase[g];
     while (bits) {
@@ -562,7 +527,7 @@ __device__ void ms_pull_body(const MsArgs& a, int ph, MsPullShared& s) {
                                 W.ev_b[n][i] = mine;
                             } else {
                                 // Out of event slots: this credit's labels now.
-                                ms_record_one(a, ph, W.vtx[i], uu[k], mine);
+                                ms_record(a, ph, W.vtx[i], uu[k], mine);
                             }
                         }
                         g |= group_or<MS_GROUP>(f);
@@ -764,7 +729,7 @@ __device__ void ms_pull_narrow_body(const MsArgs& a, int ph, MsPullNarrowShared&
                     const unsigned long long f = cur[u[k]] & want & ~got;
                     if (f) {
                         got |= f;
-                        ms_record_one(a, ph, v, u[k], f);
+                        ms_record(a, ph, v, u[k], f);
                     }
                 }
             }
@@ -795,7 +760,7 @@ __device__ void ms_pull_narrow_body(const MsArgs& a, int ph, MsPullNarrowShared&
                     if (ms_bm_test(fbm, u)) f = cur[u] & w & ~g;
                 }
                 const unsigned long long mine = f & ~group_excl_or<MS_GROUP>(f);
-                if (mine) ms_record_one(a, ph, vv, u, mine);
+                if (mine) ms_record(a, ph, vv, u, mine);
                 g |= group_or<MS_GROUP>(f);
             }
             if (sub == 0) s.got[lt] = g;
