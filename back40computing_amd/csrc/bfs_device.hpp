@@ -64,9 +64,8 @@ struct BfsArgs {
     const uint32_t* iso;     // symmetric graphs: isolated-vertex bitmap, pre-marked visited (else null)
     int32_t dev_base;        // 1: phase = ctr->ph_base + kernel argument (captured, replayed level loops)
     int32_t persistent;      // 1 inside a persistent (single-launch) search
-    int32_t coh_in;          // persistent: this phase's input frontier was written with coh_st
-    int32_t coh_out;         // persistent: this phase writes its output (queue, grid-tier entries) with coh_st
-    uint32_t light_limit;    // persistent: phases with at most this many input entries use the light barrier
+    int32_t coh_in;          // persistent: frontier inputs are read with coh_ld
+    int32_t coh_out;         // persistent: outputs (queue, grid-tier entries) are written with coh_st
     long long edges;
 };
 

@@ -1004,26 +1004,19 @@ struct FusedHeavy {
 };
 
 template <int KIND>
-__device__ void fused_body(BfsArgs a, int ph, int32_t depth, int max_levels, uint32_t sat_limit, LevelShared& s,
+__device__ void fused_body(const BfsArgs& a, int ph, int32_t depth, int max_levels, uint32_t sat_limit, LevelShared& s,
                            typename FusedHeavy<KIND>::shared& hs) {
     unsigned long long t0 = 0, busy = 0;
     if (a.wg_stats && threadIdx.x == 0) t0 = wg_clock();
     shared_init(s);
-    GridBarrier bar{a.bar_arrive, &a.ctr->bar_release, &a.ctr->bar_abort, 0u};
+    // Every phase passes its frontier through agent-scope accesses (the
+    // host sets a.persistent, a.coh_in and a.coh_out) and ends with the light
+    // barrier.  `a` is read-only here: a kernel that writes into its argument
+    // block gets a private copy of all of it, and every field read becomes a
+    // scratch load (that cost each persistent strategy a quarter of its rate).
+    const GridBarrier bar{a.bar_arrive, &a.ctr->bar_release, &a.ctr->bar_abort, 1u};
     uint32_t epoch = barrier_base_epoch(bar);
     if (blockIdx.x == 0 && threadIdx.x == 0) a.ctr->saturated = 0;
-    // Barrier mode per phase: a phase whose input holds at most light_limit
-    // entries writes its output (queue entries, grid-tier entries) with
-    // agent-scope accesses and ends with a light barrier; a wider phase
-    // writes plainly, keeping its queue traffic in L2, and ends with a
-    // fenced barrier.  Each phase reads its input in the mode it was written
-    // in.  All workgroups decide alike: the lengths come with the barriers.
-    a.persistent = 1;
-    a.coh_in = 0;  // the first input was written by an earlier launch
-    auto set_mode = [&](uint32_t n) {
-        a.coh_out = n <= a.light_limit ? 1 : 0;
-        bar.light = (uint32_t)a.coh_out;
-    };
     // Barrier payload: the length of the frontier queue in counter slot
     // `slot` (clamped to 31 bits: a frontier never holds 2^31 entries, since
     // E < 2^31) with bit 31 set when the grid tier has lists deferred by
@@ -1054,7 +1047,6 @@ __device__ void fused_body(BfsArgs a, int ph, int32_t depth, int max_levels, uin
         int heavy_ph = ph;          // phase whose deferred lists the grid tier takes
         int32_t heavy_depth = depth;  // depth label of what it discovers (top-down)
         uint32_t word = 0;
-        set_mode(n_in);
         if (KIND == FK_TD) {
             td_phase(a, ph, depth + 1, nullptr, false, s.q, s.g, s.red, gridDim.x, a.grid_th, n_in);
             heavy_depth = depth + 1;
@@ -1071,8 +1063,6 @@ __device__ void fused_body(BfsArgs a, int ph, int32_t depth, int max_levels, uin
             // has it); the overflow flag ends the search.
             uint32_t n_mid = word & 0x7FFFFFFFu;
             if (n_mid > a.qcap[(ph + 1) & 1]) n_mid = 0;
-            a.coh_in = a.coh_out;
-            set_mode(n_mid);
             tp_expand_phase(a, ph + 1, s, n_mid);
             heavy_ph = ph + 1;
             ph += 2;
@@ -1082,12 +1072,10 @@ __device__ void fused_body(BfsArgs a, int ph, int32_t depth, int max_levels, uin
         if (!grid_sync(bar, ++epoch, payload_for(ph, grid_tier ? heavy_ph : -1), &s.ctl, word)) break;
         if (word & 0x80000000u) {
             // Lists this level deferred: the whole grid takes them in
-            // chunks, appending to the same next frontier (in the level's
-            // mode: its entries were written so, and so is its output).
+            // chunks, appending to the same next frontier.
             heavy_body<FusedHeavy<KIND>::kind>(a, heavy_ph, heavy_depth, 0, hs);
             if (!grid_sync(bar, ++epoch, payload_for(ph, -1), &s.ctl, word)) break;
         }
-        a.coh_in = a.coh_out;
         n_in = word & 0x7FFFFFFFu;
         // Top-down: the level claimed vertices (at `depth`) iff the next
         // frontier is not empty (td_finish leaves this to workgroup 0).

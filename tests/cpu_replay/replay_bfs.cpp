@@ -181,9 +181,11 @@ struct Replay {
     void fused(int kind, int ph, int32_t depth_arg, uint32_t sat) {
         BfsArgs fa = a;
         if (!heavy_needed) fa.grid_th = INT_MAX;
-        // As the library: phases whose input fits one tile per workgroup use
-        // the light barrier and agent-scope frontier accesses.
-        fa.light_limit = grid * (uint32_t)BFS_THREADS;
+        // As the library: agent-scope frontier accesses and the light
+        // barrier in every phase.
+        fa.persistent = 1;
+        fa.coh_in = 1;
+        fa.coh_out = 1;
         if (kind == FK_TD)
             run<FusedLds<FK_TD>>(grid, [&](FusedLds<FK_TD>& l) {
                 fused_body<FK_TD>(fa, ph, depth_arg, 1 << 30, sat, l.s, l.hs);
