@@ -283,3 +283,36 @@ def test_query_ids_recycled_and_pack_released_first(mods, orc, lps):
     assert seen == set(range(8))
     eng.close()
 
+
+
+@pytest.mark.parametrize("per_graph", [pytest.param(2, id="narrow"), pytest.param(12, id="wide")])
+@pytest.mark.parametrize("lps", [pytest.param(0, id="launch"), pytest.param(-3, id="persistent3")])
+def test_pull_table_overflow(mods, orc, per_graph, lps):
+    """More graphs pulling bottom-up in one level than the pull's LDS graph
+    table holds (512): 600 copies of one RMAT-8 graph, each given the same
+    sources, share every direction decision (a huge alpha sends them
+    bottom-up from the first step), so each bottom-up level lists all 600
+    in one form (2 queries per graph: the narrow form, 12: the wide
+    one) and that form walks every tile of the pack with every graph's bits
+    instead.  Every query's labels must still match the oracle."""
+    bfs, batch = mods
+    g = bfs.Graph.rmat(8, 16, 0.57, 0.19, 0.19, 91)
+    off, cols = g.to_host()
+    copies = 600
+    mg = batch.MultiGraph([g] * copies)
+    assert mg.ngraphs == copies
+    srcs = [int(x) for x in orc.pick_sources(off, 92, per_graph)]
+    ref = {s: orc.bfs(off, cols, s, with_pred=False) for s in srcs}
+    eng = batch.MultiSourceBFS(mg, label_budget_bytes=1 << 28)
+    set_form(eng, lps)
+    eng.set_tuning(alpha=1 << 20)  # m_f * alpha > m_u from the first step: every level after it pulls
+    print("pull table overflow: %d graphs x %d queries, levels_per_step=%d" % (copies, per_graph, lps), flush=True)
+    res = run_all(eng, mg, [(k, s) for k in range(copies) for s in srcs])
+    assert len(res) == copies * per_graph
+    for q, ((k, s), (depth, pred, sd)) in res.items():
+        rd, _, rsd = ref[s]
+        np.testing.assert_array_equal(depth, rd)
+        assert orc.check_pred(off, cols, rd, pred) == -1
+        assert sd == rsd
+    eng.close()
+    mg.close()
