@@ -110,6 +110,29 @@ def test_gather_tiers(bfsmod, orc, sm):
             check(orc, off, cols, s, depth, pred, st)
 
 
+def test_direction_optimizing_hub_past_grid_tier_levels(bfsmod, orc):
+    """Symmetric graphs get the grid tier only on the first levels of each
+    captured block; a hub reached top-down later must be gathered inside the
+    level kernel.  A 6-edge path leads from vertex 0 to a hub of 9,000
+    leaves, behind which lies a random region large enough (with alpha 1)
+    to keep the search top-down when the hub is expanded at level 6."""
+    rng = np.random.default_rng(11)
+    n = 40000
+    edges = [(i, i + 1) for i in range(6)]
+    edges += [(6, i) for i in range(7, 9007)]
+    edges += [(9006, 9007)]
+    edges += [(int(a), int(b)) for a, b in rng.integers(9007, n, (60000, 2))]
+    off, cols = csr_from_edges(n, edges, undirected=True)
+    g = bfsmod.Graph.from_host_csr(off, cols)
+    for tuning in ({"do_alpha": 1},
+                   {"do_alpha": 1, "grid_gather_threshold": 50, "cta_gather_threshold": 20,
+                    "wave_gather_threshold": 4}):
+        for s in (0, 3, 9006):
+            depth, pred, st = run(bfsmod, g, s, b4.DIRECTION_OPTIMIZING, b4.MODE_DEFAULT, tuning=tuning)
+            check(orc, off, cols, s, depth, pred, st)
+        assert int(depth[6]) >= 0
+
+
 @pytest.mark.parametrize("sm", STRATEGY_MODES)
 def test_grid_tier_many_chunks_per_workgroup(bfsmod, orc, rmat16, sm):
     """Grid tier with a low threshold on 2 or 3 workgroups: each workgroup
